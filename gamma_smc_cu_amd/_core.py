@@ -1,0 +1,336 @@
+"""Native surface of the decoder: ctypes bindings to ``libgsmc.so``.
+
+Mirrors the attributes the reference's pybind11 module ``gamma_smc_cu._core``
+exposes on this path (``src/bindings.cpp:3497-3731``): ``FlowContext`` with
+``run_fb`` / ``run_fb_blockwise`` and the ``n_haps`` / ``n_sites`` /
+``device_id`` properties, ``cuda_mem_info``, ``bitpack`` and ``unpack``.
+``infer`` / ``infer_blockwise`` look these up as attributes at call time, so
+tests can replace them (the reference's own tests monkeypatch
+``_core.FlowContext`` and ``_core.cuda_mem_info`` the same way).
+
+The library is loaded on first use. There is no CPU fallback: without the
+built library, or without a GPU, every call raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+from .flow_field import DEFAULT_PATH
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgsmc.so")
+
+GSMC_CI = 1
+GSMC_POSTERIOR = 2
+_E_INVAL, _E_HIP, _E_NOMEM, _E_IO = -1, -2, -3, -4
+
+# Every symbol include/gamma_smc_amd.h declares.
+EXPORTS = (
+    "gsmc_last_error", "gsmc_build_tables", "gsmc_device_count", "gsmc_set_device", "gsmc_mem_info", "gsmc_bitpack",
+    "gsmc_unpack", "gsmc_context_create", "gsmc_context_create_packed", "gsmc_context_destroy",
+    "gsmc_context_info", "gsmc_context_set_tuning", "gsmc_run_fb", "gsmc_block_windows",
+    "gsmc_run_fb_blockwise", "gsmc_run_fb_device", "gsmc_malloc", "gsmc_free", "gsmc_memcpy_htod",
+    "gsmc_memcpy_dtoh", "gsmc_memset", "gsmc_synchronize", "gsmc_stream_create", "gsmc_stream_destroy",
+    "gsmc_stream_synchronize", "gsmc_event_create", "gsmc_event_destroy", "gsmc_event_record",
+    "gsmc_event_elapsed_ms",
+)
+
+_lib = None
+_lock = threading.Lock()
+
+_vp = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_i32 = ctypes.c_int
+
+
+def _declare(L):
+    L.gsmc_last_error.restype = ctypes.c_char_p
+    L.gsmc_last_error.argtypes = []
+    sig = {
+        "gsmc_build_tables": [ctypes.c_char_p, _i32, ctypes.c_double, ctypes.c_double, ctypes.c_double, _i32,
+                              _vp, _vp, _vp],
+        "gsmc_device_count": [ctypes.POINTER(ctypes.c_int)],
+        "gsmc_set_device": [_i32],
+        "gsmc_mem_info": [ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)],
+        "gsmc_bitpack": [_vp, _i64, _i64, _vp],
+        "gsmc_unpack": [_vp, _i64, _i64, _vp],
+        "gsmc_context_create": [_vp, _i64, _i64, _vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                ctypes.c_char_p, _i32, _i32, ctypes.POINTER(_vp)],
+        "gsmc_context_create_packed": [_vp, _i64, _i64, _vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_char_p, _i32, _i32, ctypes.POINTER(_vp)],
+        "gsmc_context_info": [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(ctypes.c_int),
+                              ctypes.POINTER(ctypes.c_float)],
+        "gsmc_context_set_tuning": [_vp, _i32, _i64],
+        "gsmc_run_fb": [_vp, _vp, _vp, _i64, _i32, _vp, _vp, _vp, _vp, _vp],
+        "gsmc_block_windows": [_i64, _i64, _i64, _vp, _i64, ctypes.POINTER(_i64)],
+        "gsmc_run_fb_blockwise": [_vp, _vp, _vp, _i64, _i64, _i64, _i64, _i32, _i32, _vp, _vp, _vp, _vp, _vp],
+        "gsmc_run_fb_device": [_vp, _vp, _vp, _i64, _i64, _i64, _i32, _vp, _vp, _vp, _vp, _vp, _vp],
+        "gsmc_malloc": [ctypes.POINTER(_vp), ctypes.c_uint64],
+        "gsmc_free": [_vp],
+        "gsmc_memcpy_htod": [_vp, _vp, ctypes.c_uint64],
+        "gsmc_memcpy_dtoh": [_vp, _vp, ctypes.c_uint64],
+        "gsmc_memset": [_vp, _i32, ctypes.c_uint64],
+        "gsmc_synchronize": [],
+        "gsmc_stream_create": [ctypes.POINTER(_vp)],
+        "gsmc_stream_destroy": [_vp],
+        "gsmc_stream_synchronize": [_vp],
+        "gsmc_event_create": [ctypes.POINTER(_vp)],
+        "gsmc_event_destroy": [_vp],
+        "gsmc_event_record": [_vp, _vp],
+        "gsmc_event_elapsed_ms": [_vp, _vp, ctypes.POINTER(ctypes.c_float)],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+    L.gsmc_context_destroy.argtypes = [_vp]
+    L.gsmc_context_destroy.restype = None
+
+
+def lib():
+    """The loaded C library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        with _lock:
+            if _lib is None:
+                if not os.path.exists(LIB_PATH):
+                    raise RuntimeError(
+                        f"gamma_smc_cu_amd: native library {LIB_PATH} is not built; "
+                        "run `python -c 'import __graft_entry__ as g; g.build()'` from the repository root")
+                L = ctypes.CDLL(LIB_PATH)
+                _declare(L)
+                _lib = L
+    return _lib
+
+
+def check(rc):
+    """Raise the Python exception the reference raises for the same failure."""
+    if rc == 0:
+        return
+    msg = lib().gsmc_last_error().decode(errors="replace")
+    if rc == _E_INVAL:
+        raise ValueError(msg)
+    if rc == _E_NOMEM:
+        raise MemoryError(msg)
+    raise RuntimeError(msg)
+
+
+def _ptr(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+def cuda_mem_info():
+    """``(free_bytes, total_bytes)`` of the current device (``bindings.cpp:3646-3652``)."""
+    f, t = ctypes.c_uint64(), ctypes.c_uint64()
+    check(lib().gsmc_mem_info(ctypes.byref(f), ctypes.byref(t)))
+    return int(f.value), int(t.value)
+
+
+def device_count():
+    n = ctypes.c_int()
+    check(lib().gsmc_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def set_device(device):
+    check(lib().gsmc_set_device(int(device)))
+
+
+def bitpack(G):
+    """Pack ``uint8 (n, S)`` rows into ``uint64 (n, ceil(S/64))`` on the device."""
+    G = np.ascontiguousarray(G, dtype=np.uint8)
+    if G.ndim != 2:
+        raise ValueError("G must be 2-D (n_haplotypes, n_sites)")
+    n, S = G.shape
+    out = np.zeros((n, (S + 63) // 64), np.uint64)
+    check(lib().gsmc_bitpack(_ptr(G), n, S, _ptr(out)))
+    return out
+
+
+def unpack(packed, n, S):
+    """Inverse of :func:`bitpack`."""
+    packed = np.ascontiguousarray(packed, dtype=np.uint64)
+    out = np.zeros((int(n), int(S)), np.uint8)
+    check(lib().gsmc_unpack(_ptr(packed), int(n), int(S), _ptr(out)))
+    return out
+
+
+def block_windows(n_sites, core_block_sites, flank_sites):
+    """``(n_blocks, 4)`` int32 windows, as ``FlowContext::make_block_windows``."""
+    nb = ctypes.c_int64()
+    check(lib().gsmc_block_windows(int(n_sites), int(core_block_sites), int(flank_sites), None, 0,
+                                   ctypes.byref(nb)))
+    out = np.zeros((nb.value, 4), np.int32)
+    check(lib().gsmc_block_windows(int(n_sites), int(core_block_sites), int(flank_sites), _ptr(out),
+                                   nb.value, ctypes.byref(nb)))
+    return out
+
+
+def build_tables(flow_field_path, n_steps, Ne, mu, rho, table_precision="fp32"):
+    """Host-built transition tables as the kernels read them:
+    ``fwd``/``bwd`` float32 ``(n_steps, 2, 2550, 2)`` ([step][hom, het][node][m, c])
+    and ``stretch`` ``(2550, 2)``."""
+    fwd = np.zeros((n_steps, 2, 2550, 2), np.float32)
+    bwd = np.zeros_like(fwd)
+    stretch = np.zeros((2550, 2), np.float32)
+    check(lib().gsmc_build_tables(os.fsencode(flow_field_path), int(n_steps), float(Ne), float(mu), float(rho),
+                                  _PRECISION[table_precision], _ptr(fwd), _ptr(bwd), _ptr(stretch)))
+    return fwd, bwd, stretch
+
+
+def _split_pairs(pairs):
+    pairs = list(pairs)
+    pi = np.ascontiguousarray([int(p[0]) for p in pairs], dtype=np.int32)
+    pj = np.ascontiguousarray([int(p[1]) for p in pairs], dtype=np.int32)
+    return pi, pj
+
+
+_PRECISION = {"fp32": 0, "fp64": 1}
+
+
+class FlowContext:
+    """Device context for flow-field Gamma-SMC decoding.
+
+    Same constructor arguments and methods as the reference's
+    ``_core.FlowContext`` (``bindings.cpp:2552-2670,3687-3730``), plus
+    ``table_precision`` (``"fp32"``: reference-identical table build,
+    ``"fp64"``: FP64 build) and ``packed=True`` to pass rows already
+    bitpacked by :func:`bitpack`.
+    """
+
+    def __init__(self, G, positions, Ne=10000.0, mu=1.25e-8, rho=1e-8,
+                 flow_field_path=DEFAULT_PATH, cache_steps=0, table_precision="fp32", packed=False,
+                 n_sites=None):
+        self._h = None
+        pos = np.ascontiguousarray(positions, dtype=np.float64)
+        if table_precision not in _PRECISION:
+            raise ValueError("table_precision must be 'fp32' or 'fp64'")
+        h = ctypes.c_void_p()
+        path = os.fsencode(flow_field_path)
+        if packed:
+            rows = np.ascontiguousarray(G, dtype=np.uint64)
+            n = rows.shape[0]
+            S = int(n_sites) if n_sites is not None else pos.shape[0]
+            if rows.shape[1] != (S + 63) // 64 or pos.shape[0] != S:
+                raise ValueError("packed rows / positions do not match n_sites")
+            check(lib().gsmc_context_create_packed(_ptr(rows), n, S, _ptr(pos), float(Ne), float(mu), float(rho),
+                                                   path, int(cache_steps), _PRECISION[table_precision],
+                                                   ctypes.byref(h)))
+        else:
+            G = np.ascontiguousarray(G, dtype=np.uint8)
+            if G.ndim != 2:
+                raise ValueError("G must be 2-D (n_haplotypes, n_sites)")
+            n, S = G.shape
+            if pos.shape[0] != S:
+                raise ValueError("positions length must match the number of sites in G")
+            check(lib().gsmc_context_create(_ptr(G), n, S, _ptr(pos), float(Ne), float(mu), float(rho), path,
+                                            int(cache_steps), _PRECISION[table_precision], ctypes.byref(h)))
+        self._h = h
+        n_, S_, dev, ne = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int(), ctypes.c_float()
+        check(lib().gsmc_context_info(h, ctypes.byref(n_), ctypes.byref(S_), ctypes.byref(dev), ctypes.byref(ne)))
+        self._n, self._S, self._dev, self._Ne = n_.value, S_.value, dev.value, ne.value
+
+    # properties (bindings.cpp:3727-3729)
+    @property
+    def n_haps(self):
+        return self._n
+
+    @property
+    def n_sites(self):
+        return self._S
+
+    @property
+    def device_id(self):
+        return self._dev
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h is not None and _lib is not None:
+            _lib.gsmc_context_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_tuning(self, checkpoint_k=32, min_waves_per_launch=8192):
+        check(lib().gsmc_context_set_tuning(self._h, int(checkpoint_k), int(min_waves_per_launch)))
+
+    def _alloc(self, rows, cols, mean_only, return_posterior):
+        out = {"mean": np.empty((rows, cols), np.float32)}
+        if not mean_only:
+            out["lower"] = np.empty((rows, cols), np.float32)
+            out["upper"] = np.empty((rows, cols), np.float32)
+        if return_posterior:
+            out["posterior_alpha"] = np.empty((rows, cols), np.float32)
+            out["posterior_beta"] = np.empty((rows, cols), np.float32)
+        return out
+
+    @staticmethod
+    def _flags(mean_only, return_posterior):
+        return (0 if mean_only else GSMC_CI) | (GSMC_POSTERIOR if return_posterior else 0)
+
+    def run_fb(self, pairs, mean_only=True, return_posterior=False):
+        """Full-sequence forward-backward (``bindings.cpp:2897-2989``).
+
+        Returns ``{"mean": (S, P) float32}`` plus ``lower``/``upper`` when
+        ``mean_only`` is false and ``posterior_alpha``/``posterior_beta``
+        when ``return_posterior``.
+        """
+        pi, pj = _split_pairs(pairs)
+        P = pi.shape[0]
+        if P == 0:
+            out = {"mean": np.zeros((self._S, 0), np.float32)}
+            if return_posterior:
+                out["posterior_alpha"] = np.zeros((self._S, 0), np.float32)
+                out["posterior_beta"] = np.zeros((self._S, 0), np.float32)
+            return out
+        out = self._alloc(self._S, P, mean_only, return_posterior)
+        check(lib().gsmc_run_fb(self._h, _ptr(pi), _ptr(pj), P, self._flags(mean_only, return_posterior),
+                                _ptr(out["mean"]), _ptr(out.get("lower")), _ptr(out.get("upper")),
+                                _ptr(out.get("posterior_alpha")), _ptr(out.get("posterior_beta"))))
+        return out
+
+    def run_fb_blockwise(self, pairs, core_block_sites=8192, flank_sites=2048, pair_batch_size=256,
+                         max_streams=1, mean_only=True, return_posterior=False):
+        """Blockwise forward-backward (``bindings.cpp:2991-3345``); adds ``blocks``."""
+        if core_block_sites <= 0:
+            raise ValueError("core_block_sites must be positive.")
+        if flank_sites < 0:
+            raise ValueError("flank_sites must be non-negative.")
+        if pair_batch_size == 0 or pair_batch_size < -1:
+            raise ValueError("pair_batch_size must be positive or -1 for auto.")
+        if max_streams <= 0:
+            raise ValueError("max_streams must be positive.")
+        if return_posterior and max_streams > 1:
+            raise ValueError("return_posterior=True is currently only supported with max_streams=1. "
+                             "Run blockwise posterior decoding in single-stream mode.")
+        pi, pj = _split_pairs(pairs)
+        P = pi.shape[0]
+        blocks = block_windows(self._S, core_block_sites, flank_sites)
+        if P == 0:
+            out = {"mean": np.zeros((self._S, 0), np.float32)}
+            if return_posterior:
+                out["posterior_alpha"] = np.zeros((self._S, 0), np.float32)
+                out["posterior_beta"] = np.zeros((self._S, 0), np.float32)
+            out["blocks"] = blocks
+            return out
+        out = self._alloc(self._S, P, mean_only, return_posterior)
+        if self._S:
+            check(lib().gsmc_run_fb_blockwise(self._h, _ptr(pi), _ptr(pj), P, int(core_block_sites),
+                                              int(flank_sites), int(pair_batch_size), int(max_streams),
+                                              self._flags(mean_only, return_posterior), _ptr(out["mean"]),
+                                              _ptr(out.get("lower")), _ptr(out.get("upper")),
+                                              _ptr(out.get("posterior_alpha")), _ptr(out.get("posterior_beta"))))
+        out["blocks"] = blocks
+        return out

@@ -1,0 +1,83 @@
+"""Synthetic cohorts for the decoder's workloads.
+
+Every cohort is a set of biallelic segregating sites on ``n`` haplotypes:
+
+* positions: ``S`` distinct integer coordinates drawn uniformly on
+  ``[0, span)`` and sorted;
+* derived-allele count per site: ``k`` drawn with probability proportional
+  to ``1 / k`` on ``1 .. n - 1`` (the neutral site-frequency spectrum);
+* carriers: ``k`` haplotypes chosen uniformly without replacement.
+
+Draw order for a given ``seed`` (``numpy.random.default_rng(seed)``):
+positions, then all ``k``, then carriers in site chunks of ``chunk`` sites
+(one ``random((chunk, n))`` matrix per chunk, carriers = the ``k`` smallest
+keys). The generator can emit the uint8 matrix ``G[n, S]`` or the bitpacked
+rows directly (bit ``s & 63`` of word ``s >> 6``), the latter without ever
+holding ``G`` for large cohorts.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+
+def _positions(rng, S, span):
+    if S > span:
+        raise ValueError(f"cannot place {S} distinct sites on a span of {span}")
+    pos = rng.choice(int(span), size=int(S), replace=False)
+    pos.sort()
+    return pos.astype(np.float64)
+
+
+def _counts(rng, n, S):
+    k = np.arange(1, n, dtype=np.int64)
+    p = 1.0 / k
+    p /= p.sum()
+    return rng.choice(k, size=int(S), p=p)
+
+
+def _carrier_chunks(rng, n, counts, chunk):
+    S = counts.shape[0]
+    for s0 in range(0, S, chunk):
+        s1 = min(S, s0 + chunk)
+        keys = rng.random((s1 - s0, n))
+        rank = keys.argsort(axis=1).argsort(axis=1)
+        yield s0, s1, (rank < counts[s0:s1, None])  # (chunk, n) bool
+
+
+def cohort(n, S, span, seed, chunk=8192):
+    """``(G uint8 (n, S), positions float64 (S,))`` for a 1/k-spectrum cohort."""
+    rng = np.random.default_rng(seed)
+    pos = _positions(rng, S, span)
+    counts = _counts(rng, n, S)
+    G = np.zeros((n, S), np.uint8)
+    for s0, s1, carr in _carrier_chunks(rng, n, counts, chunk):
+        G[:, s0:s1] = carr.T
+    return G, pos
+
+
+def cohort_packed(n, S, span, seed, chunk=8192):
+    """Same cohort as :func:`cohort`, returned as ``(packed uint64 (n, W), positions)``."""
+    if chunk % 64:
+        raise ValueError("chunk must be a multiple of 64")
+    rng = np.random.default_rng(seed)
+    pos = _positions(rng, S, span)
+    counts = _counts(rng, n, S)
+    W = (S + 63) // 64
+    packed = np.zeros((n, W), np.uint64)
+    weights = np.uint64(1) << np.arange(64, dtype=np.uint64)
+    for s0, s1, carr in _carrier_chunks(rng, n, counts, chunk):
+        m = s1 - s0
+        mw = (m + 63) // 64
+        bits = np.zeros((n, mw * 64), np.uint64)
+        bits[:, :m] = carr.T
+        packed[:, s0 // 64:s0 // 64 + mw] = (bits.reshape(n, mw, 64) * weights).sum(axis=2, dtype=np.uint64)
+    return packed, pos
+
+
+# Workload shapes -------------------------------------------------------
+
+CHROMOSOME = dict(n=356, S=402_853, span=51_000_000, seed=7)      # 63,190 pairs, blockwise mean_only
+BIOBANK = dict(n=1000, S=50_000, span=7_500_000, seed=17)         # 499,500 pairs
+SERVING = dict(n=6404, S=1_000_000, span=150_000_000, seed=13)    # resident cohort for the request loop
+RAGGED_COHORT = dict(n=2000, seed=11, bp_per_site=150)             # batched ragged jobs

@@ -1,0 +1,159 @@
+/*
+ * gamma_smc_amd -- C ABI of the MI355X (gfx950) Gamma-SMC decoder.
+ *
+ * Plain pointers and 64-bit sizes; no C++ or framework types cross this
+ * boundary. Every entry point returns 0 on success or a negative status
+ * (GSMC_E*) and leaves a message for gsmc_last_error() (per thread). No
+ * exception crosses the ABI. A context is bound to the device that was
+ * current when it was created; calls on one context are serialised by the
+ * context itself, calls on different contexts may run concurrently.
+ *
+ * Each entry point names the reference interface it replaces, in the
+ * reference tree of kevinkorfmann/gamma_smc_cu (file:line).
+ */
+#ifndef GAMMA_SMC_AMD_H
+#define GAMMA_SMC_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GSMC_OK 0
+#define GSMC_EINVAL -1     /* invalid argument (reference: std::invalid_argument -> ValueError) */
+#define GSMC_EHIP -2       /* HIP runtime error (reference: CUDA_CHECK -> RuntimeError)        */
+#define GSMC_ENOMEM -3     /* device or pinned host allocation failed                          */
+#define GSMC_EIO -4        /* flow-field file missing or malformed                             */
+
+/* run flags */
+#define GSMC_CI 1          /* also write Wilson-Hilferty lower / upper (mean_only=False)     */
+#define GSMC_POSTERIOR 2   /* also write posterior alpha / beta (return_posterior=True)       */
+
+/* table precision */
+#define GSMC_TABLES_F32 0  /* reference-identical FP32 table construction                    */
+#define GSMC_TABLES_F64 1  /* FP64 construction, rounded to FP32 when stored                  */
+
+typedef struct gsmc_context gsmc_context;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* gsmc_last_error(void);
+
+/* Build the transition tables on the host, without a device: fwd and bwd
+ * receive n_steps x 2 x 2550 (m, c) float pairs each ([step][hom, het][node]),
+ * stretch receives the 2550 (m, c) pairs of the n_steps-bp hom stretch.
+ * Scaled rates as gsmc_context_create forms them. Replaces
+ * build_flow_field_cache (src/flow_field.cpp:205-388) and load_flow_field
+ * (:8-54). */
+int gsmc_build_tables(const char* flow_field_path, int n_steps, double Ne, double mu, double rho,
+                      int table_precision, float* fwd, float* bwd, float* stretch);
+
+/* Number of HIP devices / select the current device (bindings.cpp:3630-3645 set_device). */
+int gsmc_device_count(int* count);
+int gsmc_set_device(int device);
+
+/* Free and total bytes of the current device.
+ * Replaces _core.cuda_mem_info (src/bindings.cpp:3646-3652). */
+int gsmc_mem_info(uint64_t* free_bytes, uint64_t* total_bytes);
+
+/* Bitpack a host uint8 matrix G[n_haps][n_sites] on the device into
+ * packed[n_haps][ceil(n_sites/64)]: bit b of word w = G[h][64w+b] & 1, tail
+ * bits zero. Replaces _core.bitpack / bitpack_genotypes_gpu
+ * (src/kernels/bitpack.cu:11-38). */
+int gsmc_bitpack(const uint8_t* G, int64_t n_haps, int64_t n_sites, uint64_t* packed);
+
+/* Inverse of gsmc_bitpack (src/kernels/bitpack.cu:43-68, _core.unpack). */
+int gsmc_unpack(const uint64_t* packed, int64_t n_haps, int64_t n_sites, uint8_t* G);
+
+/* Create a decoder context: upload and bitpack G (host uint8, row-major
+ * [n_haps][n_sites]), upload the per-site gap descriptors derived from
+ * positions (host float64 [n_sites]), and build or reuse the transition
+ * tables for (Ne, mu, rho, cache_steps) from the flow field at
+ * flow_field_path. cache_steps <= 0 means 1000. Scaled rates are
+ * 4.0f*Ne*rho and 4.0f*Ne*mu in FP32 with Ne, mu, rho cast to float.
+ * Replaces FlowContext.__init__ (src/bindings.cpp:2552-2670, binding
+ * :3687-3699) and its table set-up ensure_cache (:2016-2157). */
+int gsmc_context_create(const uint8_t* G, int64_t n_haps, int64_t n_sites, const double* positions,
+                        double Ne, double mu, double rho, const char* flow_field_path, int cache_steps,
+                        int table_precision, gsmc_context** out);
+
+/* Same, from host rows already bitpacked as gsmc_bitpack produces them
+ * (resident cohorts too large to hold as uint8). Same reference interface
+ * as gsmc_context_create, with the H2D + bitpack step (bindings.cpp:2658-2665)
+ * done by the caller. */
+int gsmc_context_create_packed(const uint64_t* packed, int64_t n_haps, int64_t n_sites,
+                               const double* positions, double Ne, double mu, double rho,
+                               const char* flow_field_path, int cache_steps, int table_precision,
+                               gsmc_context** out);
+
+/* Release a context (FlowContext destructor, src/bindings.cpp:2672-2677). */
+void gsmc_context_destroy(gsmc_context* ctx);
+
+/* n_haps, n_sites, device_id properties (src/bindings.cpp:3727-3729), and
+ * the FP32 Ne used to rescale posteriors; any pointer may be NULL. */
+int gsmc_context_info(const gsmc_context* ctx, int64_t* n_haps, int64_t* n_sites, int* device_id, float* Ne);
+
+/* Decoder tuning: checkpoint interval K (16 or 32, default 32) and the
+ * number of pair tiles x block windows one launch should hold at least
+ * (default 8192). Results do not depend on either. No reference
+ * counterpart (the reference has fixed 256-thread launches). */
+int gsmc_context_set_tuning(gsmc_context* ctx, int checkpoint_k, int64_t min_waves_per_launch);
+
+/* Full-sequence forward-backward of n_pairs pairs (pair_i[p], pair_j[p]) into
+ * host arrays laid out site-major [n_sites][n_pairs]: mean always; lower and
+ * upper when flags & GSMC_CI; alpha and beta when flags & GSMC_POSTERIOR
+ * (the others may be NULL). Replaces FlowContext.run_fb
+ * (src/bindings.cpp:2897-2989) and gamma_smc_flow_cached_fb_gpu
+ * (src/kernels/gamma_smc_flow.cu:844-903). */
+int gsmc_run_fb(gsmc_context* ctx, const int32_t* pair_i, const int32_t* pair_j, int64_t n_pairs, int flags,
+                float* mean, float* lower, float* upper, float* alpha, float* beta);
+
+/* Block windows (core_start, core_stop, padded_start, padded_stop) for
+ * n_sites sites; writes min(n_blocks, cap) rows of 4 int32 and the count.
+ * Replaces FlowContext::make_block_windows (src/bindings.cpp:2517-2534). */
+int gsmc_block_windows(int64_t n_sites, int64_t core_block_sites, int64_t flank_sites, int32_t* blocks,
+                       int64_t cap, int64_t* n_blocks);
+
+/* Blockwise forward-backward into host arrays [n_sites][n_pairs]: each
+ * padded window decoded from the prior, core rows kept. pair_batch_size > 0
+ * caps the pairs per launch (-1: all); max_streams >= 1 is the number of
+ * output staging buffers in flight. Results do not depend on either.
+ * Replaces FlowContext.run_fb_blockwise (src/bindings.cpp:2991-3345) and
+ * gamma_smc_flow_cached_fb_block_gpu[_async] (gamma_smc_flow.cu:908-997). */
+int gsmc_run_fb_blockwise(gsmc_context* ctx, const int32_t* pair_i, const int32_t* pair_j, int64_t n_pairs,
+                          int64_t core_block_sites, int64_t flank_sites, int64_t pair_batch_size,
+                          int max_streams, int flags, float* mean, float* lower, float* upper, float* alpha,
+                          float* beta);
+
+/* Device-resident form of gsmc_run_fb_blockwise: pair ids and outputs are
+ * device pointers, outputs [n_sites][n_pairs]; the work is queued on
+ * `stream` (a hipStream_t, NULL = default stream) and the call returns
+ * without synchronising. core_block_sites >= n_sites with flank 0 is the
+ * full-sequence run_fb. No allocation happens on the stream's critical path
+ * beyond the checkpoint buffer, which the context keeps between calls.
+ * Same reference interface as gsmc_run_fb_blockwise, minus the D2H. */
+int gsmc_run_fb_device(gsmc_context* ctx, const int32_t* d_pair_i, const int32_t* d_pair_j, int64_t n_pairs,
+                       int64_t core_block_sites, int64_t flank_sites, int flags, float* d_mean, float* d_lower,
+                       float* d_upper, float* d_alpha, float* d_beta, void* stream);
+
+/* Device memory, stream and event plumbing for callers without a GPU
+ * framework (bench, tests). No reference counterpart. */
+int gsmc_malloc(void** ptr, uint64_t bytes);
+int gsmc_free(void* ptr);
+int gsmc_memcpy_htod(void* dst, const void* src, uint64_t bytes);
+int gsmc_memcpy_dtoh(void* dst, const void* src, uint64_t bytes);
+int gsmc_memset(void* dst, int value, uint64_t bytes);
+int gsmc_synchronize(void);
+int gsmc_stream_create(void** stream);
+int gsmc_stream_destroy(void* stream);
+int gsmc_stream_synchronize(void* stream);
+int gsmc_event_create(void** event);
+int gsmc_event_destroy(void* event);
+int gsmc_event_record(void* event, void* stream);
+int gsmc_event_elapsed_ms(void* start, void* stop, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GAMMA_SMC_AMD_H */

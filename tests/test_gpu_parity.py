@@ -1,0 +1,144 @@
+"""GPU parity: the HIP path against the CPU oracle, through the C ABI.
+
+The oracle (``oracle/gsmc_oracle.c``) restates the reference's recursion in
+FP32 operation for operation, so the comparison is bit-exact
+(``np.array_equal`` on the raw float32 bits) for mean, CI and posterior
+arrays, for full-sequence and blockwise decoding.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+def _bits_equal(a, b):
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    return a.shape == b.shape and np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+@pytest.fixture(scope="module")
+def core():
+    from gamma_smc_cu_amd import _core
+
+    assert _core.device_count() >= 1
+    return _core
+
+
+def _tables(path, mu, rho, Ne=1e4, N=1000):
+    return O.tables_f32(path, N, Ne, mu, rho)
+
+
+def test_bitpack_roundtrip_matches_closed_form(core):
+    rng = np.random.default_rng(0)
+    for n, S in [(1, 1), (4, 1), (10, 64), (5, 128), (10, 100), (7, 1000), (3, 4097)]:
+        _case(f"bitpack n={n} S={S}")
+        G = rng.integers(0, 2, size=(n, S), dtype=np.uint8)
+        pk = core.bitpack(G)
+        assert pk.shape == (n, (S + 63) // 64)
+        np.testing.assert_array_equal(pk, O.bitpack(G))
+        np.testing.assert_array_equal(core.unpack(pk, n, S), G)
+    _case("bitpack uses bit 0 of each byte")
+    G = np.array([[2, 3, 255, 0]], np.uint8)
+    np.testing.assert_array_equal(core.unpack(core.bitpack(G), 1, 4), G & 1)
+
+
+@pytest.mark.parametrize("mean_only,posterior", [(True, False), (False, False), (True, True), (False, True)])
+def test_run_fb_bit_exact(core, cohort_small, flow_field_path, mean_only, posterior):
+    G, pos = cohort_small
+    mu, rho = O.estimate_scaled_params(G, pos, 1.25e-8, 1e-8, 1e4)
+    T = _tables(flow_field_path, mu, rho)
+    pairs = O.all_pairs(G.shape[0])
+    _case(f"run_fb mean_only={mean_only} posterior={posterior} P={len(pairs)} S={G.shape[1]}")
+    ctx = core.FlowContext(G, pos, 1e4, mu, rho, flow_field_path, 0)
+    got = ctx.run_fb(pairs, mean_only=mean_only, return_posterior=posterior)
+    ref = O.fb_window_f32(G, pos, pairs, T, 1e4, ci=not mean_only, posterior=posterior)
+    for k, v in ref.items():
+        assert _bits_equal(got[k], v), k
+
+
+@pytest.mark.parametrize("K", [16, 32])
+def test_run_fb_checkpoint_interval_invariant(core, cohort_small, flow_field_path, K):
+    G, pos = cohort_small
+    pairs = [(1, 0), (5, 2), (11, 3), (4, 4)]
+    _case(f"run_fb K={K}")
+    ctx = core.FlowContext(G, pos, 1e4, 1.25e-8, 1e-8, flow_field_path, 0)
+    ctx.set_tuning(K, 1)
+    got = ctx.run_fb(pairs, mean_only=False, return_posterior=True)
+    ref = O.fb_window_f32(G, pos, pairs, _tables(flow_field_path, 1.25e-8, 1e-8), 1e4, ci=True, posterior=True)
+    for k, v in ref.items():
+        assert _bits_equal(got[k], v), k
+
+
+@pytest.mark.parametrize("core_sites,flank", [(256, 64), (100, 37), (700, 0), (64, 300), (1, 5)])
+def test_blockwise_bit_exact(core, cohort_small, flow_field_path, core_sites, flank):
+    G, pos = cohort_small
+    pairs = [(i, j) for i in range(G.shape[0]) for j in range(i)][:50]
+    _case(f"blockwise core={core_sites} flank={flank}")
+    ctx = core.FlowContext(G, pos, 1e4, 1.25e-8, 1e-8, flow_field_path, 0)
+    got = ctx.run_fb_blockwise(pairs, core_block_sites=core_sites, flank_sites=flank, pair_batch_size=-1,
+                               max_streams=1, mean_only=False, return_posterior=True)
+    ref = O.fb_blockwise_f32(G, pos, pairs, _tables(flow_field_path, 1.25e-8, 1e-8), 1e4, core_sites, flank,
+                             ci=True, posterior=True)
+    np.testing.assert_array_equal(got["blocks"], ref["blocks"])
+    for k in ("mean", "lower", "upper", "posterior_alpha", "posterior_beta"):
+        assert _bits_equal(got[k], ref[k]), k
+
+
+def test_blockwise_pair_chunks_and_streams_invariant(core, cohort_small, flow_field_path):
+    G, pos = cohort_small
+    pairs = O.all_pairs(G.shape[0])
+    ctx = core.FlowContext(G, pos, 1e4, 1.25e-8, 1e-8, flow_field_path, 0)
+    _case("blockwise pair_batch_size=7 max_streams=3")
+    a = ctx.run_fb_blockwise(pairs, core_block_sites=128, flank_sites=32, pair_batch_size=7, max_streams=3)
+    _case("blockwise single launch")
+    b = ctx.run_fb_blockwise(pairs, core_block_sites=128, flank_sites=32, pair_batch_size=-1, max_streams=1)
+    assert _bits_equal(a["mean"], b["mean"])
+
+
+def test_gap_longer_than_cache(core, flow_field_path):
+    # gaps beyond n_max_steps exercise the stretch-layer chunking
+    rng = np.random.default_rng(5)
+    G = rng.integers(0, 2, size=(6, 40), dtype=np.uint8)
+    pos = np.cumsum(rng.choice([1, 2, 999, 1000, 1001, 2500, 7, 0], size=40)).astype(np.float64) + 1500.0
+    pairs = O.all_pairs(6)
+    _case("long gaps, cache_steps=1000")
+    ctx = core.FlowContext(G, pos, 1e4, 1.25e-8, 1e-8, flow_field_path, 0)
+    got = ctx.run_fb(pairs, mean_only=False, return_posterior=True)
+    ref = O.fb_window_f32(G, pos, pairs, _tables(flow_field_path, 1.25e-8, 1e-8), 1e4, ci=True, posterior=True)
+    for k, v in ref.items():
+        assert _bits_equal(got[k], v), k
+    _case("long gaps, cache_steps=37")
+    ctx = core.FlowContext(G, pos, 1e4, 1.25e-8, 1e-8, flow_field_path, 37)
+    got = ctx.run_fb_blockwise(pairs, core_block_sites=16, flank_sites=5, mean_only=True)
+    ref = O.fb_blockwise_f32(G, pos, pairs, _tables(flow_field_path, 1.25e-8, 1e-8, N=37), 1e4, 16, 5)
+    assert _bits_equal(got["mean"], ref["mean"])
+
+
+def test_single_site_and_empty(core, flow_field_path):
+    G = np.array([[0], [1], [1], [0]], np.uint8)
+    pos = np.array([12345.0])
+    ctx = core.FlowContext(G, pos, 1e4, 1.25e-8, 1e-8, flow_field_path, 0)
+    _case("S=1")
+    got = ctx.run_fb([(1, 0), (2, 1)], mean_only=False)
+    ref = O.fb_window_f32(G, pos, [(1, 0), (2, 1)], _tables(flow_field_path, 1.25e-8, 1e-8), 1e4, ci=True)
+    for k, v in ref.items():
+        assert _bits_equal(got[k], v), k
+    _case("P=0")
+    assert ctx.run_fb([])["mean"].shape == (1, 0)
+    r = ctx.run_fb_blockwise([], core_block_sites=8, flank_sites=2)
+    assert r["mean"].shape == (1, 0) and r["blocks"].shape == (1, 4)
+
+
+def test_pair_index_out_of_range_raises(core, cohort_small, flow_field_path):
+    G, pos = cohort_small
+    ctx = core.FlowContext(G, pos, 1e4, 1.25e-8, 1e-8, flow_field_path, 0)
+    with pytest.raises(ValueError, match="pair index out of range"):
+        ctx.run_fb([(0, G.shape[0])])
