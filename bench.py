@@ -91,7 +91,7 @@ def main():
     log(f"P={P} core={core} flank={flank} blocks={n_blocks} free={free0 / 1e9:.1f}GB")
 
     ctx = _core.FlowContext(G, pos, Ne, kmu, krho, DEFAULT_PATH, 0)
-    ctx.set_tuning(args.checkpoint_k, 8192)
+    ctx.set_tuning(args.checkpoint_k)
     d_pi = DeviceArray.from_host(pairs[:, 0].copy())
     d_pj = DeviceArray.from_host(pairs[:, 1].copy())
     d_out = DeviceArray((S, P), np.float32)

@@ -37,8 +37,18 @@ EXPORTS = (
     "gsmc_run_fb_blockwise", "gsmc_run_fb_device", "gsmc_malloc", "gsmc_free", "gsmc_memcpy_htod",
     "gsmc_memcpy_dtoh", "gsmc_memset", "gsmc_synchronize", "gsmc_stream_create", "gsmc_stream_destroy",
     "gsmc_stream_synchronize", "gsmc_event_create", "gsmc_event_destroy", "gsmc_event_record",
-    "gsmc_event_elapsed_ms",
+    "gsmc_event_elapsed_ms", "gsmc_batch_create", "gsmc_batch_destroy", "gsmc_batch_launch", "gsmc_batch_wait",
+    "gsmc_batch_query", "gsmc_checkpoint_bytes",
 )
+
+
+class Job(ctypes.Structure):
+    """``gsmc_job``: one job of a batched launch (device pointers)."""
+
+    _fields_ = [("site_lo", ctypes.c_int64), ("site_hi", ctypes.c_int64), ("core_block_sites", ctypes.c_int64),
+                ("flank_sites", ctypes.c_int64), ("pair_i", ctypes.c_void_p), ("pair_j", ctypes.c_void_p),
+                ("n_pairs", ctypes.c_int64), ("mean", ctypes.c_void_p), ("lower", ctypes.c_void_p),
+                ("upper", ctypes.c_void_p), ("alpha", ctypes.c_void_p), ("beta", ctypes.c_void_p)]
 
 _lib = None
 _lock = threading.Lock()
@@ -65,7 +75,7 @@ def _declare(L):
                                        ctypes.c_char_p, _i32, _i32, ctypes.POINTER(_vp)],
         "gsmc_context_info": [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(ctypes.c_int),
                               ctypes.POINTER(ctypes.c_float)],
-        "gsmc_context_set_tuning": [_vp, _i32, _i64],
+        "gsmc_context_set_tuning": [_vp, _i32, _i32, _i32, _i32, _i64],
         "gsmc_run_fb": [_vp, _vp, _vp, _i64, _i32, _vp, _vp, _vp, _vp, _vp],
         "gsmc_block_windows": [_i64, _i64, _i64, _vp, _i64, ctypes.POINTER(_i64)],
         "gsmc_run_fb_blockwise": [_vp, _vp, _vp, _i64, _i64, _i64, _i64, _i32, _i32, _vp, _vp, _vp, _vp, _vp],
@@ -83,6 +93,11 @@ def _declare(L):
         "gsmc_event_destroy": [_vp],
         "gsmc_event_record": [_vp, _vp],
         "gsmc_event_elapsed_ms": [_vp, _vp, ctypes.POINTER(ctypes.c_float)],
+        "gsmc_batch_create": [_vp, ctypes.POINTER(_vp)],
+        "gsmc_batch_launch": [_vp, _vp, _i64, _i32, _vp],
+        "gsmc_batch_wait": [_vp],
+        "gsmc_batch_query": [_vp, ctypes.POINTER(ctypes.c_int)],
+        "gsmc_checkpoint_bytes": [_vp, _i64, _i64, _i64, _i64, ctypes.POINTER(ctypes.c_uint64)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -90,6 +105,8 @@ def _declare(L):
         fn.restype = ctypes.c_int
     L.gsmc_context_destroy.argtypes = [_vp]
     L.gsmc_context_destroy.restype = None
+    L.gsmc_batch_destroy.argtypes = [_vp]
+    L.gsmc_batch_destroy.restype = None
 
 
 def lib():
@@ -263,8 +280,18 @@ class FlowContext:
         except Exception:
             pass
 
-    def set_tuning(self, checkpoint_k=32, min_waves_per_launch=8192):
-        check(lib().gsmc_context_set_tuning(self._h, int(checkpoint_k), int(min_waves_per_launch)))
+    def set_tuning(self, checkpoint_k=32, workgroup=64, schedule="static", xcd_order=True,
+                   min_waves_per_launch=8192):
+        """Launch shape (results are identical under every setting)."""
+        sched = {"static": 0, "persistent": 1}[schedule]
+        check(lib().gsmc_context_set_tuning(self._h, int(checkpoint_k), int(workgroup), sched, int(bool(xcd_order)),
+                                            int(min_waves_per_launch)))
+
+    def checkpoint_bytes(self, n_sites, core_block_sites, flank_sites, n_pairs):
+        out = ctypes.c_uint64()
+        check(lib().gsmc_checkpoint_bytes(self._h, int(n_sites), int(core_block_sites), int(flank_sites),
+                                          int(n_pairs), ctypes.byref(out)))
+        return int(out.value)
 
     def _alloc(self, rows, cols, mean_only, return_posterior):
         out = {"mean": np.empty((rows, cols), np.float32)}

@@ -1,8 +1,8 @@
 // Device code of the Gamma-SMC cached forward-backward decoder (gfx950).
 //
-// One lane per haplotype pair; a wave walks 64 pairs through the sites of
-// one block window in order. Per block window (core [cs, ce) padded to
-// [ps, pe)):
+// One lane per haplotype pair; a wave walks one work item -- 64 pairs of a
+// job through the sites of one block window -- in order. Per block window
+// (core [cs, ce) padded to [ps, pe)):
 //   phase 1  forward filter over [ps, ce), storing the state at the start
 //            of every K-site window of the core to HBM (pair-contiguous);
 //   phase 2  backward filter over the right flank [ce, pe) (no output);
@@ -44,27 +44,50 @@ struct DevTables {
     const char* stretch;  // one slab: N-bp hom stretch
 };
 
-struct FbArgs {
-    const uint64_t* packed;    // [n_haps][W] bitpacked rows
-    int64_t W;
-    const uint32_t* desc;      // [S] site descriptors (gap from the previous site)
-    const int32_t* pair_i;     // [P]
-    const int32_t* pair_j;     // [P]
-    int64_t P;
-    const int4* blocks;        // [n_blocks] (core_start, core_stop, padded_start, padded_stop)
-    const uint32_t* start_desc;  // [n_blocks] forward descriptor of the first padded site
-    const int64_t* ck_base;    // [n_blocks] first checkpoint window of each block
-    float2* ck;                // [windows][ckld] checkpoints, one column per lane
-    int64_t ckld;              // pair tiles * 64
-    DevTables t;
-    float unscale;             // 2 Ne
-    int64_t row0;              // output row of site row0 is 0
-    int64_t ld;                // output row stride (elements)
+// One decode job: a set of pairs over a site range of the cohort, with
+// its own outputs [rows][P] (row r = site row0 + r).
+struct JobDev {
     float* mean;
     float* lower;
     float* upper;
     float* alpha;
     float* beta;
+    const int32_t* pair_i;
+    const int32_t* pair_j;
+    int64_t P;
+    int64_t row0;
+};
+
+// One block window of a job. The walk starts from the prior at ps; the first
+// site's forward/backward descriptors are sd_fwd / sd_bwd (the site's own gap
+// descriptor at the start of a sequence, a 1-bp step at a block restart).
+struct BlockDev {
+    int32_t cs, ce, ps, pe;
+    uint32_t sd_fwd, sd_bwd;
+    int32_t job;
+    int32_t nwin;    // checkpoint windows of the core
+    int64_t ck_off;  // float2 offset of this block's checkpoints (tile-major)
+};
+
+// A work item: one 64-pair tile of one block window.
+struct ItemDev {
+    int32_t block;
+    int32_t tile;
+};
+
+struct FbArgs {
+    const uint64_t* packed;    // [n_haps][W] bitpacked rows
+    int64_t n_haps;
+    int64_t W;
+    const uint32_t* desc;      // [S] site descriptors (gap from the previous site)
+    const JobDev* jobs;
+    const BlockDev* blocks;
+    const ItemDev* items;
+    int64_t n_items;
+    int* queue;                // persistent mode: next item counter (starts at 0)
+    float2* ck;                // checkpoints: per item, [nwin][64]
+    DevTables t;
+    float unscale;             // 2 Ne
 };
 
 // Bilinear lookup of one table slab at (m, c); updates (m, c).
@@ -131,7 +154,7 @@ __device__ __forceinline__ uint32_t het_off_of(uint64_t bits, int k)
 }
 
 template <bool CI, bool POST>
-__device__ __forceinline__ void combine_store(const FbArgs& a, int64_t s, int64_t p, bool live,
+__device__ __forceinline__ void combine_store(const JobDev& j, float unscale, int64_t s, int64_t p, bool live,
                                               float fm, float fc, float ym, float yc)
 {
     const float lb = -2.0f * yc;
@@ -140,10 +163,10 @@ __device__ __forceinline__ void combine_store(const FbArgs& a, int64_t s, int64_
     const float fa = pow10_fast(lf), fb = pow10_fast(lf - fm);
     const float as = fa + ba - 1.0f;
     const float bs = fb + bb - 1.0f;
-    const float mg = (as / fmaxf(bs, 1e-10f)) * a.unscale;
+    const float mg = (as / fmaxf(bs, 1e-10f)) * unscale;
     if (!live) return;
-    const size_t idx = (size_t)(s - a.row0) * (size_t)a.ld + (size_t)p;
-    a.mean[idx] = mg;
+    const size_t idx = (size_t)(s - j.row0) * (size_t)j.P + (size_t)p;
+    j.mean[idx] = mg;
     if (CI) {
         // Wilson-Hilferty 95% bounds (gamma_smc_flow.cu:725-734).
         const float aci = fmaxf(as, 1.0f);
@@ -152,42 +175,47 @@ __device__ __forceinline__ void combine_store(const FbArgs& a, int64_t s, int64_
         const float base = 1.0f - inv9a;
         const float lo = fmaxf(fmaf(-1.96f, sq, base), 0.0f);
         const float hi = fmaf(1.96f, sq, base);
-        a.lower[idx] = fmaxf(mg * lo * lo * lo, 0.0f);
-        a.upper[idx] = mg * hi * hi * hi;
+        j.lower[idx] = fmaxf(mg * lo * lo * lo, 0.0f);
+        j.upper[idx] = mg * hi * hi * hi;
     }
     if (POST) {
-        a.alpha[idx] = as;
-        a.beta[idx] = bs;
+        j.alpha[idx] = as;
+        j.beta[idx] = bs;
     }
 }
 
-// grid: x = pair tiles of 64 lanes, y = block windows.
+// Decode one work item (64 pairs x one block window) with the calling wave.
 template <int K, bool CI, bool POST>
-__global__ __launch_bounds__(64) void fb_block_kernel(FbArgs a)
+__device__ __forceinline__ void decode_item(const FbArgs& a, int64_t item, int lane)
 {
-    static_assert(K >= 1 && K <= 64, "window must fit one het mask");
-    const int64_t p = (int64_t)blockIdx.x * 64 + threadIdx.x;
-    const bool live = p < a.P;
-    const int64_t pp = live ? p : a.P - 1;  // idle lanes shadow the last pair
-    const int4 blk = a.blocks[blockIdx.y];
-    const int64_t cs = blk.x, ce = blk.y, ps = blk.z, pe = blk.w;
-    const uint64_t* ri = a.packed + (size_t)a.pair_i[pp] * (size_t)a.W;
-    const uint64_t* rj = a.packed + (size_t)a.pair_j[pp] * (size_t)a.W;
-    const uint32_t sd = a.start_desc[blockIdx.y];
-    const bool restart = ps > 0;  // block walks start at the prior with a 1-bp first step
+    const ItemDev it = a.items[item];
+    if (it.block < 0) return;  // padding of the XCD-ordered schedule
+    const BlockDev b = a.blocks[it.block];
+    const JobDev j = a.jobs[b.job];
+    const int64_t p = (int64_t)it.tile * 64 + lane;
+    const bool live = p < j.P;
+    const int64_t pp = live ? p : j.P - 1;  // idle lanes shadow the job's last pair
+    const int64_t cs = b.cs, ce = b.ce, ps = b.ps, pe = b.pe;
+    // Pair ids from device memory are clamped to the cohort, so a bad id can
+    // only produce a wrong row, never an out-of-bounds read; the host entry
+    // points validate ids before they reach the device.
+    const int64_t hi = min(max((int64_t)j.pair_i[pp], (int64_t)0), a.n_haps - 1);
+    const int64_t hj = min(max((int64_t)j.pair_j[pp], (int64_t)0), a.n_haps - 1);
+    const uint64_t* ri = a.packed + (size_t)hi * (size_t)a.W;
+    const uint64_t* rj = a.packed + (size_t)hj * (size_t)a.W;
     // Every lane, idle ones included, owns a checkpoint column.
-    float2* ck = a.ck + (size_t)a.ck_base[blockIdx.y] * (size_t)a.ckld + (size_t)p;
+    float2* ck = a.ck + (size_t)b.ck_off + (size_t)it.tile * (size_t)b.nwin * 64 + lane;
 
     // Phase 1: forward over [ps, ce), checkpoints at the core windows.
     float m = 0.0f, c = 0.0f;
     for (int64_t s = ps; s < ce;) {
         const int64_t e = (s < cs) ? min(cs, s + K) : min(ce, s + K);
-        if (s >= cs) ck[(size_t)((s - cs) / K) * (size_t)a.ckld] = make_float2(m, c);
+        if (s >= cs) ck[(size_t)((s - cs) / K) * 64] = make_float2(m, c);
         const uint64_t bits = het_mask(ri, rj, s, a.W);
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             if (s + k < e) {
-                const uint32_t d = (k == 0 && s == ps && restart) ? sd : a.desc[s + k];
+                const uint32_t d = (k == 0 && s == ps) ? b.sd_fwd : a.desc[s + k];
                 fwd_segment(a.t, d, het_off_of(bits, k), m, c);
             }
         }
@@ -207,11 +235,10 @@ __global__ __launch_bounds__(64) void fb_block_kernel(FbArgs a)
     }
 
     // Phase 3: core windows, last to first.
-    const int64_t nwin = (ce - cs + K - 1) / K;
-    for (int64_t w = nwin - 1; w >= 0; --w) {
+    for (int64_t w = (int64_t)b.nwin - 1; w >= 0; --w) {
         const int64_t ws = cs + w * K;
         const int64_t we = min(ce, ws + K);
-        const float2 st = ck[(size_t)w * (size_t)a.ckld];
+        const float2 st = ck[(size_t)w * 64];
         m = st.x;
         c = st.y;
         const uint64_t bits = het_mask(ri, rj, ws, a.W);
@@ -219,7 +246,7 @@ __global__ __launch_bounds__(64) void fb_block_kernel(FbArgs a)
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             if (ws + k < we) {
-                const uint32_t d = (k == 0 && ws == ps && restart) ? sd : a.desc[ws + k];
+                const uint32_t d = (k == 0 && ws == ps) ? b.sd_fwd : a.desc[ws + k];
                 fwd_segment(a.t, d, het_off_of(bits, k), m, c);
                 fm[k] = m;
                 fc[k] = c;
@@ -228,13 +255,39 @@ __global__ __launch_bounds__(64) void fb_block_kernel(FbArgs a)
 #pragma unroll
         for (int k = K - 1; k >= 0; --k) {
             if (ws + k < we) {
-                combine_store<CI, POST>(a, ws + k, p, live, fm[k], fc[k], ym, yc);
-                const uint32_t d = (k == 0 && ws == ps && restart) ? 0u : a.desc[ws + k];
+                combine_store<CI, POST>(j, a.unscale, ws + k, p, live, fm[k], fc[k], ym, yc);
+                const uint32_t d = (k == 0 && ws == ps) ? b.sd_bwd : a.desc[ws + k];
                 bwd_segment(a.t, d, het_off_of(bits, k), ym, yc);
             }
         }
     }
 }
+
+// Static schedule: wave w of workgroup g decodes item g * (WG / 64) + w.
+template <int K, bool CI, bool POST, int WG>
+__global__ __launch_bounds__(WG) void fb_items_kernel(FbArgs a)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t item = (int64_t)blockIdx.x * (WG / 64) + (threadIdx.x >> 6);
+    if (item < a.n_items) decode_item<K, CI, POST>(a, item, lane);
+}
+
+// Persistent schedule: each wave takes items from a device counter until
+// the queue is drained; every wave exits once the counter passes n_items.
+template <int K, bool CI, bool POST, int WG>
+__global__ __launch_bounds__(WG) void fb_queue_kernel(FbArgs a)
+{
+    const int lane = threadIdx.x & 63;
+    for (;;) {
+        int item = 0;
+        if (lane == 0) item = atomicAdd(a.queue, 1);
+        item = __shfl(item, 0);
+        if (item >= a.n_items) break;
+        decode_item<K, CI, POST>(a, item, lane);
+    }
+}
+
+__global__ void reset_counter_kernel(int* counter) { *counter = 0; }
 
 // Bitpack: one wave per (haplotype, 64-site word); the ballot of the 64
 // lanes' bits is the word (bit b of word w = G[h][64 w + b] & 1, tail 0).

@@ -225,42 +225,77 @@ uint32_t descriptor(int steps, int N)
 
 using namespace gsmc;
 
+struct Tuning {
+    int K = 32;              // checkpoint interval = register window
+    int wg = 64;             // workgroup size (64 or 256 lanes)
+    int schedule = 0;        // 0 static, 1 persistent queue
+    int xcd_order = 1;       // static schedule: give each XCD a contiguous run of items
+    int64_t min_waves = 8192;
+};
+
+struct gsmc_batch;
+
 struct gsmc_context {
     std::mutex mu;
     int device = 0;
+    int cus = 0;
     int64_t n = 0, S = 0, W = 0;
     float Ne = 0.0f;
     int N = 1000;
-    int K = 32;
-    int64_t min_waves = 8192;
+    Tuning tune;
     std::vector<double> pos;
     DevBuf packed, desc;
     std::shared_ptr<DeviceTables> tabs;
-    // launch scratch of the host-output path
-    DevBuf ck, meta, pairs;
-    // launch scratch of the device-resident path
-    DevBuf dev_ck, dev_meta;
-    std::vector<char> dev_meta_host;
+    // host-output path
+    DevBuf pairs;
     DevBuf out_slot[2];
     PinnedBuf stage[2];
     hipStream_t compute = nullptr, copy = nullptr;
     hipEvent_t ev_kernel[2] = {nullptr, nullptr};  // slot written (compute -> copy)
     hipEvent_t ev_free[2] = {nullptr, nullptr};    // slot drained (copy -> compute)
     hipEvent_t ev_stage[2] = {nullptr, nullptr};   // staging buffer filled (copy -> host)
+    std::unique_ptr<gsmc_batch> host_batch[2];
+    std::unique_ptr<gsmc_batch> dev_batch;
 
-    ~gsmc_context()
+    ~gsmc_context();
+};
+
+// A launch workspace: metadata, checkpoints and queue counter of one launch
+// in flight. A new launch on the workspace first waits for the previous one
+// unless it repeats the same metadata on the same stream.
+struct gsmc_batch {
+    gsmc_context* ctx = nullptr;
+    DevBuf meta, ck, queue;
+    PinnedBuf stage;
+    std::vector<char> last;
+    hipStream_t last_stream = nullptr;
+    hipEvent_t done = nullptr;
+    bool pending = false;
+    ~gsmc_batch()
     {
-        (void)hipSetDevice(device);
-        for (auto& e : ev_kernel)
-            if (e) (void)hipEventDestroy(e);
-        for (auto& e : ev_free)
-            if (e) (void)hipEventDestroy(e);
-        for (auto& e : ev_stage)
-            if (e) (void)hipEventDestroy(e);
-        if (compute) (void)hipStreamDestroy(compute);
-        if (copy) (void)hipStreamDestroy(copy);
+        if (done) {
+            (void)hipSetDevice(ctx->device);
+            if (pending) (void)hipEventSynchronize(done);
+            (void)hipEventDestroy(done);
+        }
     }
 };
+
+gsmc_context::~gsmc_context()
+{
+    (void)hipSetDevice(device);
+    dev_batch.reset();
+    host_batch[0].reset();
+    host_batch[1].reset();
+    for (auto& e : ev_kernel)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& e : ev_free)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& e : ev_stage)
+        if (e) (void)hipEventDestroy(e);
+    if (compute) (void)hipStreamDestroy(compute);
+    if (copy) (void)hipStreamDestroy(copy);
+}
 
 namespace gsmc {
 namespace {
@@ -277,6 +312,14 @@ void init_streams(gsmc_context* ctx)
     }
 }
 
+std::unique_ptr<gsmc_batch> new_batch(gsmc_context* ctx)
+{
+    auto b = std::make_unique<gsmc_batch>();
+    b->ctx = ctx;
+    HIP_OK(hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
+    return b;
+}
+
 void finish_create(gsmc_context* ctx, const double* positions, double Ne, double mu, double rho, const char* path,
                    int cache_steps, int precision)
 {
@@ -291,6 +334,9 @@ void finish_create(gsmc_context* ctx, const double* positions, double Ne, double
     }
     ctx->desc.ensure(d.size() * 4, ctx->device);
     HIP_OK(hipMemcpy(ctx->desc.p, d.data(), d.size() * 4, hipMemcpyHostToDevice));
+    hipDeviceProp_t prop;
+    HIP_OK(hipGetDeviceProperties(&prop, ctx->device));
+    ctx->cus = prop.multiProcessorCount;
 }
 
 struct Block {
@@ -316,81 +362,186 @@ struct Outputs {
     float* beta;
 };
 
-// Launch metadata of one decode: block windows (int4), first checkpoint
-// window per block (int64), forward descriptor of each block's first padded
-// site (uint32). `windows` is the launch's checkpoint row count.
-struct Meta {
-    std::vector<char> bytes;
-    int64_t nb = 0;
-    int64_t windows = 0;
+// One job of a launch: pairs (device ids) decoded over `blocks` (global site
+// windows) of the sequence that starts at site seq_lo; outputs are
+// [rows][P] with row r = site row0 + r.
+struct JobSpec {
+    int64_t seq_lo = 0;
+    std::vector<Block> blocks;
+    const int32_t* pi = nullptr;
+    const int32_t* pj = nullptr;
+    int64_t P = 0;
+    Outputs out{};
+    int64_t row0 = 0;
 };
 
-Meta make_meta(const gsmc_context* ctx, const Block* blocks, int64_t nb)
+// Launch metadata: [jobs][blocks][items], 256-byte aligned sections.
+struct Plan {
+    std::vector<char> bytes;
+    size_t off_blocks = 0, off_items = 0;
+    int64_t n_items = 0;
+    int64_t n_waves = 0;  // waves with work (items without padding)
+    size_t ck_elems = 0;
+};
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+Plan make_plan(const gsmc_context* ctx, const std::vector<JobSpec>& jobs, const Tuning& tu)
 {
-    require(nb <= 65535, "too many block windows for one launch");
-    Meta m;
-    m.nb = nb;
-    m.bytes.resize((size_t)nb * 28);
-    std::vector<int64_t> base((size_t)nb);
-    std::vector<uint32_t> sdesc((size_t)nb);
-    for (int64_t b = 0; b < nb; ++b) {
-        base[b] = m.windows;
-        m.windows += (blocks[b].ce - blocks[b].cs + ctx->K - 1) / ctx->K;
-        const double pps = ctx->pos[blocks[b].ps];
-        sdesc[b] = descriptor(round_steps(pps - (pps - 1.0)), ctx->N);
+    std::vector<JobDev> jd;
+    std::vector<BlockDev> bd;
+    std::vector<ItemDev> items;
+    size_t ck = 0;
+    for (size_t j = 0; j < jobs.size(); ++j) {
+        const JobSpec& js = jobs[j];
+        if (js.P == 0 || js.blocks.empty()) continue;
+        jd.push_back({js.out.mean, js.out.lower, js.out.upper, js.out.alpha, js.out.beta, js.pi, js.pj, js.P,
+                      js.row0});
+        const int64_t tiles = (js.P + 63) / 64;
+        for (const Block& b : js.blocks) {
+            BlockDev d;
+            d.cs = b.cs;
+            d.ce = b.ce;
+            d.ps = b.ps;
+            d.pe = b.pe;
+            if (b.ps == js.seq_lo) {
+                // first site of the decoded sequence: its gap from position -1
+                d.sd_fwd = d.sd_bwd = descriptor(round_steps(ctx->pos[b.ps] + 1.0), ctx->N);
+            } else {
+                // block restart: one bp from (position - 1), as the block kernels do
+                const double pp = ctx->pos[b.ps];
+                d.sd_fwd = descriptor(round_steps(pp - (pp - 1.0)), ctx->N);
+                d.sd_bwd = 0;
+            }
+            d.job = (int32_t)(jd.size() - 1);
+            d.nwin = (int32_t)((b.ce - b.cs + tu.K - 1) / tu.K);
+            d.ck_off = (int64_t)ck;
+            ck += (size_t)tiles * d.nwin * 64;
+            require(bd.size() < (size_t)INT32_MAX, "too many block windows");
+            for (int64_t t = 0; t < tiles; ++t) items.push_back({(int32_t)bd.size(), (int32_t)t});
+            bd.push_back(d);
+        }
     }
-    std::memcpy(m.bytes.data(), blocks, (size_t)nb * 16);
-    std::memcpy(m.bytes.data() + (size_t)nb * 16, base.data(), (size_t)nb * 8);
-    std::memcpy(m.bytes.data() + (size_t)nb * 24, sdesc.data(), (size_t)nb * 4);
-    return m;
+    Plan pl;
+    pl.n_waves = (int64_t)items.size();
+    if (tu.schedule == 0 && tu.xcd_order && !items.empty()) {
+        // Workgroup g runs on XCD g % 8: hand each XCD a contiguous run of
+        // workgroup-sized chunks, so the waves sharing an XCD's L2 walk the
+        // same blocks (and so the same table layers) at the same time.
+        const int64_t wpw = tu.wg / 64;
+        const int64_t G = ((int64_t)items.size() + wpw - 1) / wpw;
+        const int64_t per = (G + 7) / 8;
+        std::vector<ItemDev> perm((size_t)(per * 8 * wpw), ItemDev{-1, 0});
+        for (int64_t g = 0; g < per * 8; ++g) {
+            const int64_t chunk = (g % 8) * per + g / 8;
+            for (int64_t w = 0; w < wpw; ++w) {
+                const int64_t src = chunk * wpw + w;
+                if (chunk < G && src < (int64_t)items.size()) perm[(size_t)(g * wpw + w)] = items[(size_t)src];
+            }
+        }
+        items.swap(perm);
+    }
+    pl.n_items = (int64_t)items.size();
+    pl.ck_elems = ck;
+    const size_t sj = jd.size() * sizeof(JobDev), sb = bd.size() * sizeof(BlockDev);
+    pl.off_blocks = align256(sj);
+    pl.off_items = pl.off_blocks + align256(sb);
+    pl.bytes.assign(pl.off_items + items.size() * sizeof(ItemDev), 0);
+    if (sj) std::memcpy(pl.bytes.data(), jd.data(), sj);
+    if (sb) std::memcpy(pl.bytes.data() + pl.off_blocks, bd.data(), sb);
+    if (!items.empty()) std::memcpy(pl.bytes.data() + pl.off_items, items.data(), items.size() * sizeof(ItemDev));
+    return pl;
 }
 
-// Queue one launch over nb blocks whose metadata sits at d_meta, for pairs
-// d_pi/d_pj (device) into outputs whose row r holds site row0 + r with row
-// stride ld. The checkpoint buffer must hold windows x ckld float2.
-void launch_fb(gsmc_context* ctx, const char* d_meta, int64_t nb, float2* ck, const int32_t* d_pi,
-               const int32_t* d_pj, int64_t P, int flags, const Outputs& out, int64_t row0, int64_t ld,
-               hipStream_t stream)
+template <int K, bool C, bool Q, int WG>
+void launch_kernel(const FbArgs& a, int schedule, unsigned grid, hipStream_t st)
 {
-    if (P == 0 || nb == 0) return;
-    const int64_t tiles = (P + 63) / 64;
+    if (schedule == 1)
+        hipLaunchKernelGGL((fb_queue_kernel<K, C, Q, WG>), dim3(grid), dim3(WG), 0, st, a);
+    else
+        hipLaunchKernelGGL((fb_items_kernel<K, C, Q, WG>), dim3(grid), dim3(WG), 0, st, a);
+}
+
+template <int K, int WG>
+void dispatch_flags(const FbArgs& a, int flags, int schedule, unsigned grid, hipStream_t st)
+{
+    const bool ci = flags & GSMC_CI, post = flags & GSMC_POSTERIOR;
+    if (ci && post) launch_kernel<K, true, true, WG>(a, schedule, grid, st);
+    else if (ci) launch_kernel<K, true, false, WG>(a, schedule, grid, st);
+    else if (post) launch_kernel<K, false, true, WG>(a, schedule, grid, st);
+    else launch_kernel<K, false, false, WG>(a, schedule, grid, st);
+}
+
+// Waves of the decode kernel one device holds at once (persistent grid size).
+int64_t resident_waves(const gsmc_context* ctx, const Tuning& tu, int flags)
+{
+    int per_cu = 0;
+    const bool ci = flags & GSMC_CI, post = flags & GSMC_POSTERIOR;
+    const void* fn = nullptr;
+#define GSMC_FN(KK, WW)                                                                        \
+    fn = ci ? (post ? (const void*)fb_queue_kernel<KK, true, true, WW> : (const void*)fb_queue_kernel<KK, true, false, WW>) \
+            : (post ? (const void*)fb_queue_kernel<KK, false, true, WW> : (const void*)fb_queue_kernel<KK, false, false, WW>);
+    if (tu.K == 16) {
+        if (tu.wg == 256) { GSMC_FN(16, 256) } else { GSMC_FN(16, 64) }
+    } else {
+        if (tu.wg == 256) { GSMC_FN(32, 256) } else { GSMC_FN(32, 64) }
+    }
+#undef GSMC_FN
+    HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, tu.wg, 0));
+    return (int64_t)std::max(per_cu, 1) * ctx->cus * (tu.wg / 64);
+}
+
+// Queue one launch of `jobs` on `stream` using workspace b.
+void launch_plan(gsmc_batch* b, const std::vector<JobSpec>& jobs, int flags, hipStream_t stream)
+{
+    gsmc_context* ctx = b->ctx;
+    const Tuning tu = ctx->tune;
+    Plan pl = make_plan(ctx, jobs, tu);
+    if (pl.n_items == 0) return;
+    const bool same = b->pending && stream == b->last_stream && pl.bytes == b->last;
+    if (!same) {
+        if (b->pending) HIP_OK(hipEventSynchronize(b->done));
+        b->pending = false;
+        b->meta.ensure(pl.bytes.size(), ctx->device);
+        b->ck.ensure(std::max<size_t>(pl.ck_elems, 1) * sizeof(float2), ctx->device);
+        b->queue.ensure(256, ctx->device);
+        b->stage.ensure(pl.bytes.size());
+        std::memcpy(b->stage.p, pl.bytes.data(), pl.bytes.size());
+        HIP_OK(hipMemcpyAsync(b->meta.p, b->stage.p, pl.bytes.size(), hipMemcpyHostToDevice, stream));
+        b->last.swap(pl.bytes);
+        b->last_stream = stream;
+    }
     FbArgs a;
     a.packed = ctx->packed.as<uint64_t>();
+    a.n_haps = ctx->n;
     a.W = ctx->W;
     a.desc = ctx->desc.as<uint32_t>();
-    a.pair_i = d_pi;
-    a.pair_j = d_pj;
-    a.P = P;
-    a.blocks = reinterpret_cast<const int4*>(d_meta);
-    a.ck_base = reinterpret_cast<const int64_t*>(d_meta + (size_t)nb * 16);
-    a.start_desc = reinterpret_cast<const uint32_t*>(d_meta + (size_t)nb * 24);
-    a.ck = ck;
-    a.ckld = tiles * 64;
+    a.jobs = reinterpret_cast<const JobDev*>(b->meta.p);
+    a.blocks = reinterpret_cast<const BlockDev*>(b->meta.as<char>() + pl.off_blocks);
+    a.items = reinterpret_cast<const ItemDev*>(b->meta.as<char>() + pl.off_items);
+    a.n_items = pl.n_items;
+    a.queue = b->queue.as<int>();
+    a.ck = b->ck.as<float2>();
     a.t = ctx->tabs->view;
     a.unscale = 2.0f * ctx->Ne;
-    a.row0 = row0;
-    a.ld = ld;
-    a.mean = out.mean;
-    a.lower = out.lower;
-    a.upper = out.upper;
-    a.alpha = out.alpha;
-    a.beta = out.beta;
-    const dim3 grid((unsigned)tiles, (unsigned)nb);
-    const bool ci = flags & GSMC_CI, post = flags & GSMC_POSTERIOR;
-#define GSMC_LAUNCH(KK, C, Q) hipLaunchKernelGGL((fb_block_kernel<KK, C, Q>), grid, dim3(64), 0, stream, a)
-#define GSMC_BY_FLAGS(KK)                        \
-    if (ci && post) GSMC_LAUNCH(KK, true, true);   \
-    else if (ci) GSMC_LAUNCH(KK, true, false);     \
-    else if (post) GSMC_LAUNCH(KK, false, true);   \
-    else GSMC_LAUNCH(KK, false, false);
-    if (ctx->K == 16) {
-        GSMC_BY_FLAGS(16)
+    const int64_t wpw = tu.wg / 64;
+    unsigned grid;
+    if (tu.schedule == 1) {
+        hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, stream, a.queue);
+        grid = (unsigned)std::max<int64_t>(1, std::min(resident_waves(ctx, tu, flags), pl.n_items) / wpw);
     } else {
-        GSMC_BY_FLAGS(32)
+        grid = (unsigned)((pl.n_items + wpw - 1) / wpw);
     }
-#undef GSMC_BY_FLAGS
-#undef GSMC_LAUNCH
+    if (tu.K == 16) {
+        if (tu.wg == 256) dispatch_flags<16, 256>(a, flags, tu.schedule, grid, stream);
+        else dispatch_flags<16, 64>(a, flags, tu.schedule, grid, stream);
+    } else {
+        if (tu.wg == 256) dispatch_flags<32, 256>(a, flags, tu.schedule, grid, stream);
+        else dispatch_flags<32, 64>(a, flags, tu.schedule, grid, stream);
+    }
     HIP_OK(hipGetLastError());
+    HIP_OK(hipEventRecord(b->done, stream));
+    b->pending = true;
 }
 
 int n_outputs(int flags) { return 1 + ((flags & GSMC_CI) ? 2 : 0) + ((flags & GSMC_POSTERIOR) ? 2 : 0); }
@@ -430,22 +581,23 @@ void scatter_rows(float* dst, int64_t ld_dst, int64_t col0, const float* src, in
 // device slot, then drained to host through pinned staging while the next
 // group runs.
 void run_to_host(gsmc_context* ctx, const int32_t* pi, const int32_t* pj, int64_t P, const std::vector<Block>& blocks,
-                 int64_t pair_batch, int slots, int flags, float* const* host_out)
+                 int64_t pair_batch, int flags, float* const* host_out)
 {
     init_streams(ctx);
+    for (auto& hb : ctx->host_batch)
+        if (!hb) hb = new_batch(ctx);
     const int nout = n_outputs(flags);
     const int64_t chunk = (pair_batch > 0) ? std::min<int64_t>(pair_batch, P) : P;
-    // Blocks per launch: enough pair-tile x block waves to fill the chip,
-    // bounded by a device budget for the output slots.
+    // Blocks per launch: enough tile x block waves to fill the chip, bounded
+    // by a device budget for the two output slots.
     size_t free_b = 0, total_b = 0;
     HIP_OK(hipMemGetInfo(&free_b, &total_b));
     const int64_t tiles = (chunk + 63) / 64;
-    int64_t per_group = std::max<int64_t>(1, (ctx->min_waves + tiles - 1) / tiles);
+    int64_t per_group = std::max<int64_t>(1, (ctx->tune.min_waves + tiles - 1) / tiles);
     int64_t max_rows = 0;
     for (const Block& b : blocks) max_rows = std::max<int64_t>(max_rows, b.ce - b.cs);
-    const size_t slot_budget = (size_t)(free_b * 0.30);
+    const size_t slot_budget = (size_t)(free_b * 0.25);
     while (per_group > 1 && (size_t)per_group * max_rows * chunk * 4 * nout > slot_budget) per_group /= 2;
-    slots = std::max(1, std::min(slots, 2));
 
     ctx->pairs.ensure((size_t)P * 8, ctx->device);
     int32_t* d_pi = ctx->pairs.as<int32_t>();
@@ -464,39 +616,18 @@ void run_to_host(gsmc_context* ctx, const int32_t* pi, const int32_t* pj, int64_
             const int64_t r0 = blocks[b0].cs, r1 = blocks[b0 + nb - 1].ce;
             tasks.push_back({p0, std::min(chunk, P - p0), b0, nb, r0, r1 - r0});
         }
-
-    // Metadata of every launch, uploaded once; the largest checkpoint and
-    // slot sizes reserved up front, so nothing is reallocated or overwritten
-    // while a queued launch can still read it.
-    std::vector<size_t> meta_off(tasks.size());
-    {
-        std::vector<char> all;
-        size_t ck_max = 0;
-        for (size_t i = 0; i < tasks.size(); ++i) {
-            const Task& t = tasks[i];
-            Meta m = make_meta(ctx, blocks.data() + t.b0, t.nb);
-            meta_off[i] = all.size();
-            all.insert(all.end(), m.bytes.begin(), m.bytes.end());
-            all.resize((all.size() + 255) & ~(size_t)255);
-            ck_max = std::max(ck_max, (size_t)m.windows * (size_t)((t.np + 63) / 64 * 64) * sizeof(float2));
-        }
-        ctx->ck.ensure(ck_max, ctx->device);
-        ctx->meta.ensure(all.size(), ctx->device);
-        HIP_OK(hipStreamSynchronize(ctx->compute));
-        HIP_OK(hipMemcpy(ctx->meta.p, all.data(), all.size(), hipMemcpyHostToDevice));
-        for (int s = 0; s < slots; ++s) {
-            size_t slot_max = 0;
-            for (size_t i = s; i < tasks.size(); i += slots)
-                slot_max = std::max(slot_max, (size_t)tasks[i].rows * tasks[i].np * 4 * nout);
-            ctx->out_slot[s].ensure(slot_max, ctx->device);
-        }
+    for (int s = 0; s < 2; ++s) {
+        size_t slot_max = 0;
+        for (size_t i = s; i < tasks.size(); i += 2)
+            slot_max = std::max(slot_max, (size_t)tasks[i].rows * tasks[i].np * 4 * nout);
+        if (slot_max) ctx->out_slot[s].ensure(slot_max, ctx->device);
     }
+
     bool slot_used[2] = {false, false};
     const size_t stage_bytes = (size_t)256 << 20;
     for (int s = 0; s < 2; ++s) ctx->stage[s].ensure(stage_bytes);
     int stage_next = 0;
     bool stage_busy[2] = {false, false};
-    // pending host copies per staging buffer
     struct Pending {
         int out;
         int64_t row, rows, col0, cols;
@@ -511,9 +642,9 @@ void run_to_host(gsmc_context* ctx, const int32_t* pi, const int32_t* pj, int64_
         stage_busy[s] = false;
     };
 
-    // copy one finished slot out, through the staging ring, in row pieces
+    // copy one finished slot out through the staging ring, in row pieces
     auto copy_out = [&](int slot, const Task& t) {
-        const size_t plane = (size_t)t.rows * t.np;  // floats per output array in the slot
+        const size_t plane = (size_t)t.rows * t.np;
         const int64_t rows_per_piece = std::max<int64_t>(1, (int64_t)(stage_bytes / 4 / std::max<int64_t>(t.np, 1)));
         HIP_OK(hipStreamWaitEvent(ctx->copy, ctx->ev_kernel[slot], 0));
         for (int o = 0; o < nout; ++o) {
@@ -537,32 +668,31 @@ void run_to_host(gsmc_context* ctx, const int32_t* pi, const int32_t* pj, int64_
     const Task* prev_task = nullptr;
     for (size_t i = 0; i < tasks.size(); ++i) {
         const Task& t = tasks[i];
-        const int slot = (int)(i % (size_t)slots);
+        const int slot = (int)(i % 2);
         const size_t plane = (size_t)t.rows * t.np;
-        // the slot's previous contents were fully staged before we reuse it
-        if (slots == 1 && prev_slot >= 0) {
-            copy_out(prev_slot, *prev_task);
-            prev_slot = -1;
-            for (int s = 0; s < 2; ++s) drain_stage(s);
-        }
         if (slot_used[slot]) HIP_OK(hipStreamWaitEvent(ctx->compute, ctx->ev_free[slot], 0));
         slot_used[slot] = true;
         float* base = ctx->out_slot[slot].as<float>();
-        Outputs o{};
+        JobSpec js;
+        js.seq_lo = 0;
+        js.blocks.assign(blocks.begin() + t.b0, blocks.begin() + t.b0 + t.nb);
+        js.pi = d_pi + t.p0;
+        js.pj = d_pj + t.p0;
+        js.P = t.np;
+        js.row0 = t.row0;
         int k = 0;
-        o.mean = base + plane * k++;
+        js.out.mean = base + plane * k++;
         if (flags & GSMC_CI) {
-            o.lower = base + plane * k++;
-            o.upper = base + plane * k++;
+            js.out.lower = base + plane * k++;
+            js.out.upper = base + plane * k++;
         }
         if (flags & GSMC_POSTERIOR) {
-            o.alpha = base + plane * k++;
-            o.beta = base + plane * k++;
+            js.out.alpha = base + plane * k++;
+            js.out.beta = base + plane * k++;
         }
-        launch_fb(ctx, ctx->meta.as<char>() + meta_off[i], t.nb, ctx->ck.as<float2>(), d_pi + t.p0, d_pj + t.p0,
-                  t.np, flags, o, t.row0, t.np, ctx->compute);
+        launch_plan(ctx->host_batch[slot].get(), {js}, flags, ctx->compute);
         HIP_OK(hipEventRecord(ctx->ev_kernel[slot], ctx->compute));
-        if (slots > 1 && prev_slot >= 0) copy_out(prev_slot, *prev_task);
+        if (prev_slot >= 0) copy_out(prev_slot, *prev_task);
         prev_slot = slot;
         prev_task = &t;
     }
@@ -674,6 +804,7 @@ static int create_common(const uint8_t* G, const uint64_t* packed, int64_t n, in
         HIP_OK(hipGetDevice(&ctx->device));
         ctx->n = n;
         ctx->S = S;
+        require(n <= INT32_MAX, "n_haps too large");
         ctx->W = (S + 63) / 64;
         const size_t pk_bytes = (size_t)n * ctx->W * 8;
         ctx->packed.ensure(pk_bytes, ctx->device);
@@ -736,15 +867,21 @@ int gsmc_context_info(const gsmc_context* ctx, int64_t* n, int64_t* S, int* devi
     });
 }
 
-int gsmc_context_set_tuning(gsmc_context* ctx, int k, int64_t min_waves)
+int gsmc_context_set_tuning(gsmc_context* ctx, int k, int workgroup, int schedule, int xcd_order,
+                            int64_t min_waves)
 {
     return guarded([&] {
         require(ctx != nullptr, "context is NULL");
         require(k == 16 || k == 32, "checkpoint_k must be 16 or 32");
+        require(workgroup == 64 || workgroup == 256, "workgroup must be 64 or 256");
+        require(schedule == 0 || schedule == 1, "schedule must be 0 (static) or 1 (persistent)");
         require(min_waves >= 1, "min_waves_per_launch must be positive");
         std::lock_guard<std::mutex> lk(ctx->mu);
-        ctx->K = k;
-        ctx->min_waves = min_waves;
+        ctx->tune.K = k;
+        ctx->tune.wg = workgroup;
+        ctx->tune.schedule = schedule;
+        ctx->tune.xcd_order = xcd_order ? 1 : 0;
+        ctx->tune.min_waves = min_waves;
     });
 }
 
@@ -770,11 +907,10 @@ int gsmc_run_fb(gsmc_context* ctx, const int32_t* pi, const int32_t* pj, int64_t
     return guarded([&] {
         require(ctx != nullptr, "context is NULL");
         require(P >= 0, "negative n_pairs");
-        require(mean != nullptr || P * ctx->S == 0, "mean is NULL");
-        require(!(flags & GSMC_CI) || (lower && upper) || P * ctx->S == 0, "lower/upper required with GSMC_CI");
-        require(!(flags & GSMC_POSTERIOR) || (alpha && beta) || P * ctx->S == 0,
-                "alpha/beta required with GSMC_POSTERIOR");
         if (P == 0 || ctx->S == 0) return;
+        require(mean != nullptr, "mean is NULL");
+        require(!(flags & GSMC_CI) || (lower && upper), "lower/upper required with GSMC_CI");
+        require(!(flags & GSMC_POSTERIOR) || (alpha && beta), "alpha/beta required with GSMC_POSTERIOR");
         check_pairs(ctx, pi, pj, P);
         std::lock_guard<std::mutex> lk(ctx->mu);
         HIP_OK(hipSetDevice(ctx->device));
@@ -784,7 +920,7 @@ int gsmc_run_fb(gsmc_context* ctx, const int32_t* pi, const int32_t* pj, int64_t
         outs[k++] = mean;
         if (flags & GSMC_CI) outs[k++] = lower, outs[k++] = upper;
         if (flags & GSMC_POSTERIOR) outs[k++] = alpha, outs[k++] = beta;
-        run_to_host(ctx, pi, pj, P, blocks, -1, 2, flags, outs);
+        run_to_host(ctx, pi, pj, P, blocks, -1, flags, outs);
     });
 }
 
@@ -815,7 +951,7 @@ int gsmc_run_fb_blockwise(gsmc_context* ctx, const int32_t* pi, const int32_t* p
         outs[k++] = mean;
         if (flags & GSMC_CI) outs[k++] = lower, outs[k++] = upper;
         if (flags & GSMC_POSTERIOR) outs[k++] = alpha, outs[k++] = beta;
-        run_to_host(ctx, pi, pj, P, blocks, pair_batch, std::max(2, max_streams), flags, outs);
+        run_to_host(ctx, pi, pj, P, blocks, pair_batch, flags, outs);
     });
 }
 
@@ -834,21 +970,106 @@ int gsmc_run_fb_device(gsmc_context* ctx, const int32_t* d_pi, const int32_t* d_
         require(!(flags & GSMC_POSTERIOR) || (d_alpha && d_beta), "alpha/beta required with GSMC_POSTERIOR");
         std::lock_guard<std::mutex> lk(ctx->mu);
         HIP_OK(hipSetDevice(ctx->device));
-        auto blocks = make_blocks(ctx->S, core, flank);
-        Meta m = make_meta(ctx, blocks.data(), (int64_t)blocks.size());
-        const size_t ck_bytes = (size_t)m.windows * (size_t)((P + 63) / 64 * 64) * sizeof(float2);
-        // Metadata and checkpoints are reused while the launch shape repeats;
-        // a new shape waits for queued work before replacing them.
-        if (m.bytes != ctx->dev_meta_host || ck_bytes > ctx->dev_ck.bytes) {
-            HIP_OK(hipDeviceSynchronize());
-            ctx->dev_meta.ensure(m.bytes.size(), ctx->device);
-            ctx->dev_ck.ensure(ck_bytes, ctx->device);
-            HIP_OK(hipMemcpy(ctx->dev_meta.p, m.bytes.data(), m.bytes.size(), hipMemcpyHostToDevice));
-            ctx->dev_meta_host = m.bytes;
+        if (!ctx->dev_batch) ctx->dev_batch = new_batch(ctx);
+        JobSpec js;
+        js.blocks = make_blocks(ctx->S, core, flank);
+        js.pi = d_pi;
+        js.pj = d_pj;
+        js.P = P;
+        js.out = Outputs{d_mean, d_lower, d_upper, d_alpha, d_beta};
+        launch_plan(ctx->dev_batch.get(), {js}, flags, static_cast<hipStream_t>(stream));
+    });
+}
+
+int gsmc_batch_create(gsmc_context* ctx, gsmc_batch** out)
+{
+    return guarded([&] {
+        require(ctx != nullptr && out != nullptr, "NULL argument");
+        HIP_OK(hipSetDevice(ctx->device));
+        *out = new_batch(ctx).release();
+    });
+}
+
+void gsmc_batch_destroy(gsmc_batch* b) { delete b; }
+
+int gsmc_batch_launch(gsmc_batch* b, const gsmc_job* jobs, int64_t n_jobs, int flags, void* stream)
+{
+    return guarded([&] {
+        require(b != nullptr, "batch is NULL");
+        require(n_jobs >= 0 && (jobs != nullptr || n_jobs == 0), "bad job list");
+        gsmc_context* ctx = b->ctx;
+        std::vector<JobSpec> specs;
+        specs.reserve((size_t)n_jobs);
+        for (int64_t i = 0; i < n_jobs; ++i) {
+            const gsmc_job& j = jobs[i];
+            require(j.site_lo >= 0 && j.site_lo <= j.site_hi && j.site_hi <= ctx->S, "job site range out of bounds");
+            require(j.n_pairs >= 0, "negative n_pairs");
+            require(j.flank_sites >= 0, "flank_sites must be non-negative.");
+            if (j.n_pairs == 0 || j.site_hi == j.site_lo) continue;
+            require(j.mean != nullptr && j.pair_i != nullptr && j.pair_j != nullptr, "job pointers are NULL");
+            require(!(flags & GSMC_CI) || (j.lower && j.upper), "lower/upper required with GSMC_CI");
+            require(!(flags & GSMC_POSTERIOR) || (j.alpha && j.beta), "alpha/beta required with GSMC_POSTERIOR");
+            const int64_t len = j.site_hi - j.site_lo;
+            JobSpec js;
+            js.seq_lo = j.site_lo;
+            js.blocks = make_blocks(len, j.core_block_sites > 0 ? j.core_block_sites : len,
+                                    j.core_block_sites > 0 ? j.flank_sites : 0);
+            for (Block& bk : js.blocks) {
+                bk.cs += (int32_t)j.site_lo;
+                bk.ce += (int32_t)j.site_lo;
+                bk.ps += (int32_t)j.site_lo;
+                bk.pe += (int32_t)j.site_lo;
+            }
+            js.pi = j.pair_i;
+            js.pj = j.pair_j;
+            js.P = j.n_pairs;
+            js.out = Outputs{j.mean, j.lower, j.upper, j.alpha, j.beta};
+            js.row0 = j.site_lo;
+            specs.push_back(std::move(js));
         }
-        Outputs o{d_mean, d_lower, d_upper, d_alpha, d_beta};
-        launch_fb(ctx, ctx->dev_meta.as<char>(), m.nb, ctx->dev_ck.as<float2>(), d_pi, d_pj, P, flags, o, 0, P,
-                  static_cast<hipStream_t>(stream));
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        HIP_OK(hipSetDevice(ctx->device));
+        launch_plan(b, specs, flags, static_cast<hipStream_t>(stream));
+    });
+}
+
+int gsmc_batch_wait(gsmc_batch* b)
+{
+    return guarded([&] {
+        require(b != nullptr, "batch is NULL");
+        if (b->pending) HIP_OK(hipEventSynchronize(b->done));
+        b->pending = false;
+    });
+}
+
+int gsmc_batch_query(gsmc_batch* b, int* done)
+{
+    return guarded([&] {
+        require(b != nullptr && done != nullptr, "NULL argument");
+        if (!b->pending) {
+            *done = 1;
+            return;
+        }
+        const hipError_t e = hipEventQuery(b->done);
+        if (e == hipErrorNotReady) {
+            *done = 0;
+            return;
+        }
+        HIP_OK(e);
+        *done = 1;
+    });
+}
+
+int gsmc_checkpoint_bytes(gsmc_context* ctx, int64_t n_sites, int64_t core, int64_t flank, int64_t n_pairs,
+                          uint64_t* bytes)
+{
+    return guarded([&] {
+        require(ctx != nullptr && bytes != nullptr, "NULL argument");
+        const int64_t tiles = (n_pairs + 63) / 64;
+        uint64_t windows = 0;
+        for (const Block& b : make_blocks(n_sites, core > 0 ? core : std::max<int64_t>(n_sites, 1), core > 0 ? flank : 0))
+            windows += (uint64_t)((b.ce - b.cs + ctx->tune.K - 1) / ctx->tune.K);
+        *bytes = windows * (uint64_t)tiles * 64 * sizeof(float2);
     });
 }
 

@@ -35,6 +35,27 @@ extern "C" {
 #define GSMC_TABLES_F64 1  /* FP64 construction, rounded to FP32 when stored                  */
 
 typedef struct gsmc_context gsmc_context;
+typedef struct gsmc_batch gsmc_batch;
+
+/* One job of a batched launch: pairs decoded over sites [site_lo, site_hi)
+ * of the context as one sequence (its first site's gap measured from
+ * position -1, as for a cohort holding only those sites), split into block
+ * windows of core_block_sites with flank_sites flanks (core_block_sites <= 0:
+ * one window). Pointers are device memory; outputs are
+ * [site_hi - site_lo][n_pairs]. Pair ids are read on the device without a
+ * host check: ids outside [0, n_haps) are clamped and give undefined rows. */
+typedef struct gsmc_job {
+    int64_t site_lo, site_hi;
+    int64_t core_block_sites, flank_sites;
+    const int32_t* pair_i;
+    const int32_t* pair_j;
+    int64_t n_pairs;
+    float* mean;
+    float* lower;
+    float* upper;
+    float* alpha;
+    float* beta;
+} gsmc_job;
 
 /* Message of the last failed call on this thread ("" if none). */
 const char* gsmc_last_error(void);
@@ -93,11 +114,15 @@ void gsmc_context_destroy(gsmc_context* ctx);
  * the FP32 Ne used to rescale posteriors; any pointer may be NULL. */
 int gsmc_context_info(const gsmc_context* ctx, int64_t* n_haps, int64_t* n_sites, int* device_id, float* Ne);
 
-/* Decoder tuning: checkpoint interval K (16 or 32, default 32) and the
- * number of pair tiles x block windows one launch should hold at least
- * (default 8192). Results do not depend on either. No reference
- * counterpart (the reference has fixed 256-thread launches). */
-int gsmc_context_set_tuning(gsmc_context* ctx, int checkpoint_k, int64_t min_waves_per_launch);
+/* Decoder tuning; results do not depend on any of it. checkpoint_k: sites
+ * per checkpoint window, 16 or 32 (default 32); workgroup: 64 or 256 lanes
+ * (default 64); schedule: 0 static grid of work items, 1 persistent waves
+ * draining a device queue (default 0); xcd_order: static grid ordered so
+ * each XCD walks a contiguous run of items (default 1); min_waves: work
+ * items the host-output path puts in one launch at least (default 8192).
+ * No reference counterpart (the reference has fixed 256-thread launches). */
+int gsmc_context_set_tuning(gsmc_context* ctx, int checkpoint_k, int workgroup, int schedule, int xcd_order,
+                            int64_t min_waves_per_launch);
 
 /* Full-sequence forward-backward of n_pairs pairs (pair_i[p], pair_j[p]) into
  * host arrays laid out site-major [n_sites][n_pairs]: mean always; lower and
@@ -135,6 +160,25 @@ int gsmc_run_fb_blockwise(gsmc_context* ctx, const int32_t* pair_i, const int32_
 int gsmc_run_fb_device(gsmc_context* ctx, const int32_t* d_pair_i, const int32_t* d_pair_j, int64_t n_pairs,
                        int64_t core_block_sites, int64_t flank_sites, int flags, float* d_mean, float* d_lower,
                        float* d_upper, float* d_alpha, float* d_beta, void* stream);
+
+/* Launch workspaces for batched ragged jobs and the serving loop: a batch
+ * owns the metadata, checkpoint and queue buffers of one launch in flight.
+ * gsmc_batch_launch queues every job of the list as one kernel launch on
+ * `stream` (work items = 64-pair tiles x block windows of all jobs) and
+ * returns; a later launch on the same batch first waits for this one.
+ * gsmc_batch_query reports completion without blocking, gsmc_batch_wait
+ * blocks. No reference counterpart (the reference decodes one request per
+ * launch sequence). */
+int gsmc_batch_create(gsmc_context* ctx, gsmc_batch** out);
+void gsmc_batch_destroy(gsmc_batch* batch);
+int gsmc_batch_launch(gsmc_batch* batch, const gsmc_job* jobs, int64_t n_jobs, int flags, void* stream);
+int gsmc_batch_wait(gsmc_batch* batch);
+int gsmc_batch_query(gsmc_batch* batch, int* done);
+
+/* Device bytes of checkpoints one job of n_pairs pairs over n_sites sites
+ * needs under the context's tuning (for admission control). */
+int gsmc_checkpoint_bytes(gsmc_context* ctx, int64_t n_sites, int64_t core_block_sites, int64_t flank_sites,
+                          int64_t n_pairs, uint64_t* bytes);
 
 /* Device memory, stream and event plumbing for callers without a GPU
  * framework (bench, tests). No reference counterpart. */

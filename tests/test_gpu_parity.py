@@ -70,7 +70,7 @@ def test_run_fb_checkpoint_interval_invariant(core, cohort_small, flow_field_pat
     pairs = [(1, 0), (5, 2), (11, 3), (4, 4)]
     _case(f"run_fb K={K}")
     ctx = core.FlowContext(G, pos, 1e4, 1.25e-8, 1e-8, flow_field_path, 0)
-    ctx.set_tuning(K, 1)
+    ctx.set_tuning(K, min_waves_per_launch=1)
     got = ctx.run_fb(pairs, mean_only=False, return_posterior=True)
     ref = O.fb_window_f32(G, pos, pairs, _tables(flow_field_path, 1.25e-8, 1e-8), 1e4, ci=True, posterior=True)
     for k, v in ref.items():
