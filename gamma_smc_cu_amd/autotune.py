@@ -1,0 +1,54 @@
+"""Launch-shape autotuner for the decode kernel.
+
+Every candidate computes bit-identical results (the launch shape decides
+only which wave decodes which 64-pair tile of which block window, and how
+many sites a checkpoint window spans), so the tuner is free to pick the
+fastest. Candidates sweep:
+
+* ``checkpoint_k`` -- sites per checkpoint window, which is also the length
+  of the forward window recomputed into registers (8 or 16);
+* ``workgroup`` -- 64 or 256 lanes (1 or 4 pair tiles per workgroup);
+* ``occupancy`` -- no floor, or (K = 8) a kernel compiled to fit 6 waves per
+  SIMD (fewer registers for the two interleaved lookup chains, more waves);
+* ``schedule`` -- a static grid of work items, ordered per XCD or not, or
+  persistent waves draining a device queue.
+
+The core block size is not a tuning knob: block windows change the
+posterior (each window restarts from the prior), so they stay what the
+caller asked for.
+"""
+
+from __future__ import annotations
+
+import itertools
+import time
+
+CANDIDATES = [
+    dict(checkpoint_k=k, workgroup=wg, occupancy=occ, schedule=sch, xcd_order=xcd)
+    for k, occ, wg, (sch, xcd) in itertools.product(
+        (8, 16), (1, 6), (64, 256), (("static", False), ("static", True), ("persistent", False)))
+    if occ == 1 or k == 8  # the occupancy floor is built for K = 8 only
+]
+
+
+def tune(ctx, run_once, candidates=None, repeats=1, log=None):
+    """Time ``run_once()`` (which launches one decode on ``ctx`` and returns
+    its duration in ms) under each candidate; leave ``ctx`` set to the
+    fastest and return ``(best, [(candidate, ms), ...])``."""
+    results = []
+    for cand in candidates or CANDIDATES:
+        ctx.set_tuning(**cand)
+        run_once()  # first launch of a shape uploads metadata and grows buffers
+        ms = min(run_once() for _ in range(max(1, repeats)))
+        results.append((cand, ms))
+        if log:
+            log(f"tune {cand} -> {ms:.2f} ms")
+    best = min(results, key=lambda r: r[1])[0]
+    ctx.set_tuning(**best)
+    return best, results
+
+
+def timed(fn):
+    t0 = time.perf_counter()
+    fn()
+    return (time.perf_counter() - t0) * 1e3

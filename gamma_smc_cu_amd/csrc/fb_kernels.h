@@ -38,6 +38,31 @@ constexpr float kCvStep = (0.0f - -2.0f) / 49;
 constexpr float kMeanStepInv = 1.0f / kMeanStep;
 constexpr float kCvStepInv = 1.0f / kCvStep;
 
+// Address-space views. Descriptors and launch metadata are read-only while a
+// kernel runs, so they may go through the scalar (constant) cache; output
+// and checkpoint pointers are global memory, which lets the compiler use
+// a scalar base plus a per-lane offset instead of 64-bit flat addresses.
+typedef __attribute__((address_space(4))) const uint32_t const_u32;
+typedef __attribute__((address_space(1))) char global_char;
+
+// Read-only metadata through the scalar cache, one dword at a time (the
+// compiler merges the dwords into wide scalar loads).
+template <class T> __device__ __forceinline__ T load_const(const T* p)
+{
+    static_assert(sizeof(T) % 4 == 0, "metadata records are dword multiples");
+    T v;
+    const const_u32* q = (const const_u32*)p;
+    uint32_t* o = reinterpret_cast<uint32_t*>(&v);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) o[i] = q[i];
+    return v;
+}
+
+template <class T> __device__ __forceinline__ T* global_at(void* base, uint32_t byte_off)
+{
+    return (T*)((global_char*)base + byte_off);
+}
+
 struct DevTables {
     const char* fwd;      // [N] layers of kLayerBytes
     const char* bwd;      // [N] layers of kLayerBytes
@@ -102,7 +127,7 @@ __device__ __forceinline__ void lookup(const char* __restrict__ layer, uint32_t 
     const int c0 = min((int)fc, 48);
     const float wm = fm - (float)m0;
     const float wc = fc - (float)c0;
-    const uint32_t off = het_off + (uint32_t)(m0 * 50 + c0) * 8u;
+    const uint32_t off = het_off + __umul24((uint32_t)m0, 400u) + (uint32_t)c0 * 8u;
     const float2 v00 = *reinterpret_cast<const float2*>(layer + off);
     const float2 v01 = *reinterpret_cast<const float2*>(layer + off + 8);
     const float2 v10 = *reinterpret_cast<const float2*>(layer + off + 400);
@@ -148,25 +173,44 @@ __device__ __forceinline__ uint64_t het_mask(const uint64_t* __restrict__ ri, co
     return (x0 >> sh) | (x1 << (64 - sh));
 }
 
+// K consecutive site descriptors from s (the array is padded by 64 entries
+// past the last site), through scalar loads.
+template <int K>
+__device__ __forceinline__ void load_descs(const uint32_t* desc, int64_t s, uint32_t (&d)[K])
+{
+    const const_u32* p = (const const_u32*)(desc + s);
+#pragma unroll
+    for (int k = 0; k < K; ++k) d[k] = p[k];
+}
+
 __device__ __forceinline__ uint32_t het_off_of(uint64_t bits, int k)
 {
     return ((uint32_t)(bits >> k) & 1u) * kSlabBytes;
 }
 
+// Streaming store that bypasses cache retention (outputs are never re-read).
+__device__ __forceinline__ void store_stream(float* row, uint32_t col, float v)
+{
+    __builtin_nontemporal_store(v, global_at<float>(row, col));
+}
+
 template <bool CI, bool POST>
-__device__ __forceinline__ void combine_store(const JobDev& j, float unscale, int64_t s, int64_t p, bool live,
-                                              float fm, float fc, float ym, float yc)
+__device__ __forceinline__ float combine_mean(float fm, float fc, float ym, float yc, float unscale, float& as,
+                                              float& bs)
 {
     const float lb = -2.0f * yc;
     const float ba = pow10_fast(lb), bb = pow10_fast(lb - ym);
     const float lf = -2.0f * fc;
     const float fa = pow10_fast(lf), fb = pow10_fast(lf - fm);
-    const float as = fa + ba - 1.0f;
-    const float bs = fb + bb - 1.0f;
-    const float mg = (as / fmaxf(bs, 1e-10f)) * unscale;
-    if (!live) return;
-    const size_t idx = (size_t)(s - j.row0) * (size_t)j.P + (size_t)p;
-    j.mean[idx] = mg;
+    as = fa + ba - 1.0f;
+    bs = fb + bb - 1.0f;
+    return (as / fmaxf(bs, 1e-10f)) * unscale;
+}
+
+// The CI / posterior outputs of one site, stored at once.
+template <bool CI, bool POST>
+__device__ __forceinline__ void store_extras(const JobDev& j, size_t row, uint32_t col, float mg, float as, float bs)
+{
     if (CI) {
         // Wilson-Hilferty 95% bounds (gamma_smc_flow.cu:725-734).
         const float aci = fmaxf(as, 1.0f);
@@ -175,26 +219,97 @@ __device__ __forceinline__ void combine_store(const JobDev& j, float unscale, in
         const float base = 1.0f - inv9a;
         const float lo = fmaxf(fmaf(-1.96f, sq, base), 0.0f);
         const float hi = fmaf(1.96f, sq, base);
-        j.lower[idx] = fmaxf(mg * lo * lo * lo, 0.0f);
-        j.upper[idx] = mg * hi * hi * hi;
+        store_stream(j.lower + row, col, fmaxf(mg * lo * lo * lo, 0.0f));
+        store_stream(j.upper + row, col, mg * hi * hi * hi);
     }
     if (POST) {
-        j.alpha[idx] = as;
-        j.beta[idx] = bs;
+        store_stream(j.alpha + row, col, as);
+        store_stream(j.beta + row, col, bs);
+    }
+}
+
+// True when none of the K descriptors carries a full-stretch chunk (gap
+// beyond the table depth): the window's lookups are then one table layer
+// each and need no loop.
+template <int K> __device__ __forceinline__ bool no_long_gaps(const uint32_t (&d)[K])
+{
+    uint32_t any = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) any |= d[k];
+    return (any >> kDescLayerBits) == 0;
+}
+
+__device__ __forceinline__ const char* layer_of(const char* base, uint32_t d)
+{
+    return base + (size_t)(d & kDescLayerMask) * kLayerBytes;
+}
+
+// Phase-3 step over one checkpoint window, software-pipelined: the
+// backward walk of window w (cnt sites at ws, forward states in F) runs in
+// the same loop as the forward recompute of window w - 1 (K sites at
+// ws - K) when has_next. The two chains are independent, so each wave
+// keeps two lookups in flight. The recomputed states reuse the slots the
+// backward walk frees, which flips the slot order every window:
+// orientation A holds site ws + k in slot k, orientation B in slot K-1-k.
+// Window outputs are buffered and stored after the loop, so no lookup of
+// the window waits on an output store. FAST (a full window and a full next
+// window, no long gaps in either) has no branch inside the loop, so the
+// compiler can interleave the two chains' instructions.
+template <int K, bool CI, bool POST, bool B, bool FAST>
+__device__ __forceinline__ void window_step(const FbArgs& a, const JobDev& j, int64_t ws, int64_t cnt, bool has_next,
+                                            const uint32_t (&dc)[K], uint64_t bc, const uint32_t (&dn)[K],
+                                            uint64_t bn, float m, float c, uint32_t col, bool live, float (&fm)[K],
+                                            float (&fc)[K], float& ym, float& yc)
+{
+    float out[K], oa[K], ob[K];
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+        const int k = K - 1 - i;
+        const int slot = B ? i : k;  // current window: site ws + k
+        if (FAST || k < cnt) {
+            out[k] = combine_mean<CI, POST>(fm[slot], fc[slot], ym, yc, a.unscale, oa[k], ob[k]);
+            if (FAST)
+                lookup(layer_of(a.t.bwd, dc[k]), het_off_of(bc, k), ym, yc);
+            else
+                bwd_segment(a.t, dc[k], het_off_of(bc, k), ym, yc);
+        }
+        if (FAST || has_next) {
+            if (FAST)
+                lookup(layer_of(a.t.fwd, dn[i]), het_off_of(bn, i), m, c);
+            else
+                fwd_segment(a.t, dn[i], het_off_of(bn, i), m, c);
+            fm[slot] = m;  // next window: site ws - K + i, in the slot just freed
+            fc[slot] = c;
+        }
+    }
+    if (live) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if (FAST || k < cnt) {
+                const size_t row = (size_t)(ws + k - j.row0) * (size_t)j.P;
+                store_stream(j.mean + row, col, out[k]);
+                if (CI || POST) store_extras<CI, POST>(j, row, col, out[k], oa[k], ob[k]);
+            }
+        }
     }
 }
 
 // Decode one work item (64 pairs x one block window) with the calling wave.
 template <int K, bool CI, bool POST>
-__device__ __forceinline__ void decode_item(const FbArgs& a, int64_t item, int lane)
+__device__ __forceinline__ void decode_item(const FbArgs& a, int item_in, int lane)
 {
-    const ItemDev it = a.items[item];
+    // The item is the same for the whole wave; saying so keeps every piece
+    // of block and job metadata, and each site's descriptor, in scalar
+    // registers and scalar loads.
+    const int item = __builtin_amdgcn_readfirstlane(item_in);
+    const ItemDev it = load_const(a.items + item);
     if (it.block < 0) return;  // padding of the XCD-ordered schedule
-    const BlockDev b = a.blocks[it.block];
-    const JobDev j = a.jobs[b.job];
+    const BlockDev b = load_const(a.blocks + it.block);
+    const JobDev j = load_const(a.jobs + b.job);
     const int64_t p = (int64_t)it.tile * 64 + lane;
     const bool live = p < j.P;
     const int64_t pp = live ? p : j.P - 1;  // idle lanes shadow the job's last pair
+    const uint32_t col = (uint32_t)p * 4u;
     const int64_t cs = b.cs, ce = b.ce, ps = b.ps, pe = b.pe;
     // Pair ids from device memory are clamped to the cohort, so a bad id can
     // only produce a wrong row, never an out-of-bounds read; the host entry
@@ -203,21 +318,23 @@ __device__ __forceinline__ void decode_item(const FbArgs& a, int64_t item, int l
     const int64_t hj = min(max((int64_t)j.pair_j[pp], (int64_t)0), a.n_haps - 1);
     const uint64_t* ri = a.packed + (size_t)hi * (size_t)a.W;
     const uint64_t* rj = a.packed + (size_t)hj * (size_t)a.W;
-    // Every lane, idle ones included, owns a checkpoint column.
-    float2* ck = a.ck + (size_t)b.ck_off + (size_t)it.tile * (size_t)b.nwin * 64 + lane;
+    // Every lane, idle ones included, owns a checkpoint column: window w of
+    // this item lives at ck_tile[w * 64 + lane].
+    float2* ck_tile = a.ck + (size_t)b.ck_off + (size_t)it.tile * (size_t)b.nwin * 64;
+    const uint32_t ck_lane = (uint32_t)lane * 8u;
 
     // Phase 1: forward over [ps, ce), checkpoints at the core windows.
     float m = 0.0f, c = 0.0f;
     for (int64_t s = ps; s < ce;) {
         const int64_t e = (s < cs) ? min(cs, s + K) : min(ce, s + K);
-        if (s >= cs) ck[(size_t)((s - cs) / K) * 64] = make_float2(m, c);
+        if (s >= cs) *global_at<float2>(ck_tile + (size_t)((s - cs) / K) * 64, ck_lane) = make_float2(m, c);
         const uint64_t bits = het_mask(ri, rj, s, a.W);
+        uint32_t d[K];
+        load_descs<K>(a.desc, s, d);
+        if (s == ps) d[0] = b.sd_fwd;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            if (s + k < e) {
-                const uint32_t d = (k == 0 && s == ps) ? b.sd_fwd : a.desc[s + k];
-                fwd_segment(a.t, d, het_off_of(bits, k), m, c);
-            }
+            if (s + k < e) fwd_segment(a.t, d[k], het_off_of(bits, k), m, c);
         }
         s = e;
     }
@@ -227,67 +344,113 @@ __device__ __forceinline__ void decode_item(const FbArgs& a, int64_t item, int l
     for (int64_t e = pe; e > ce;) {
         const int64_t b0 = max(ce, e - K);
         const uint64_t bits = het_mask(ri, rj, b0, a.W);
+        uint32_t d[K];
+        load_descs<K>(a.desc, b0, d);
 #pragma unroll
         for (int k = K - 1; k >= 0; --k) {
-            if (b0 + k < e) bwd_segment(a.t, a.desc[b0 + k], het_off_of(bits, k), ym, yc);
+            if (b0 + k < e) bwd_segment(a.t, d[k], het_off_of(bits, k), ym, yc);
         }
         e = b0;
     }
 
-    // Phase 3: core windows, last to first.
-    for (int64_t w = (int64_t)b.nwin - 1; w >= 0; --w) {
+    // Phase 3: core windows, last to first. The last window (possibly
+    // partial) is recomputed alone into orientation A; every later
+    // recompute rides along the previous window's backward walk.
+    float fm[K], fc[K];
+    int64_t w = (int64_t)b.nwin - 1;
+    int64_t cnt = ce - (cs + w * K);
+    {
         const int64_t ws = cs + w * K;
-        const int64_t we = min(ce, ws + K);
-        const float2 st = ck[(size_t)w * 64];
+        const float2 st = *global_at<float2>(ck_tile + (size_t)w * 64, ck_lane);
         m = st.x;
         c = st.y;
         const uint64_t bits = het_mask(ri, rj, ws, a.W);
-        float fm[K], fc[K];
+        uint32_t d[K];
+        load_descs<K>(a.desc, ws, d);
+        if (ws == ps) d[0] = b.sd_fwd;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            if (ws + k < we) {
-                const uint32_t d = (k == 0 && ws == ps) ? b.sd_fwd : a.desc[ws + k];
-                fwd_segment(a.t, d, het_off_of(bits, k), m, c);
+            if (k < cnt) {
+                fwd_segment(a.t, d[k], het_off_of(bits, k), m, c);
                 fm[k] = m;
                 fc[k] = c;
             }
         }
+    }
+    bool orient_b = false;
+    for (;;) {
+        const bool has_next = w > 0;
+        const int64_t ws = cs + w * K;
+        uint32_t dc[K], dn[K];
+        load_descs<K>(a.desc, ws, dc);
+        if (ws == ps) dc[0] = b.sd_bwd;
+        const uint64_t bc = het_mask(ri, rj, ws, a.W);
+        uint64_t bn = 0;
+        const int64_t wn = ws - K;
+        if (has_next) {
+            const float2 st = *global_at<float2>(ck_tile + (size_t)(w - 1) * 64, ck_lane);
+            m = st.x;
+            c = st.y;
+            load_descs<K>(a.desc, wn, dn);
+            if (wn == ps) dn[0] = b.sd_fwd;
+            bn = het_mask(ri, rj, wn, a.W);
+        } else {
 #pragma unroll
-        for (int k = K - 1; k >= 0; --k) {
-            if (ws + k < we) {
-                combine_store<CI, POST>(j, a.unscale, ws + k, p, live, fm[k], fc[k], ym, yc);
-                const uint32_t d = (k == 0 && ws == ps) ? b.sd_bwd : a.desc[ws + k];
-                bwd_segment(a.t, d, het_off_of(bits, k), ym, yc);
-            }
+            for (int k = 0; k < K; ++k) dn[k] = 0;
         }
+        const bool fast = has_next && cnt == K && no_long_gaps<K>(dc) && no_long_gaps<K>(dn);
+        if (fast) {
+            if (orient_b)
+                window_step<K, CI, POST, true, true>(a, j, ws, cnt, true, dc, bc, dn, bn, m, c, col, live, fm, fc, ym, yc);
+            else
+                window_step<K, CI, POST, false, true>(a, j, ws, cnt, true, dc, bc, dn, bn, m, c, col, live, fm, fc, ym, yc);
+        } else {
+            if (orient_b)
+                window_step<K, CI, POST, true, false>(a, j, ws, cnt, has_next, dc, bc, dn, bn, m, c, col, live, fm, fc, ym, yc);
+            else
+                window_step<K, CI, POST, false, false>(a, j, ws, cnt, has_next, dc, bc, dn, bn, m, c, col, live, fm, fc, ym, yc);
+        }
+        if (!has_next) break;
+        orient_b = !orient_b;
+        --w;
+        cnt = K;
     }
 }
 
 // Static schedule: wave w of workgroup g decodes item g * (WG / 64) + w.
-template <int K, bool CI, bool POST, int WG>
-__global__ __launch_bounds__(WG) void fb_items_kernel(FbArgs a)
+// OCC asks the compiler to fit at least OCC waves per SIMD (1: no floor).
+template <int K, bool CI, bool POST, int WG, int OCC>
+__global__ __launch_bounds__(WG, OCC) void fb_items_kernel(FbArgs a)
 {
     const int lane = threadIdx.x & 63;
-    const int64_t item = (int64_t)blockIdx.x * (WG / 64) + (threadIdx.x >> 6);
+    const int item = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (WG / 64) + (threadIdx.x >> 6)));
     if (item < a.n_items) decode_item<K, CI, POST>(a, item, lane);
 }
 
 // Persistent schedule: each wave takes items from a device counter until
 // the queue is drained; every wave exits once the counter passes n_items.
-template <int K, bool CI, bool POST, int WG>
-__global__ __launch_bounds__(WG) void fb_queue_kernel(FbArgs a)
+template <int K, bool CI, bool POST, int WG, int OCC>
+__global__ __launch_bounds__(WG, OCC) void fb_queue_kernel(FbArgs a)
 {
     const int lane = threadIdx.x & 63;
     for (;;) {
         int item = 0;
         if (lane == 0) item = atomicAdd(a.queue, 1);
-        item = __shfl(item, 0);
+        item = __builtin_amdgcn_readfirstlane(__shfl(item, 0));
         if (item >= a.n_items) break;
         decode_item<K, CI, POST>(a, item, lane);
     }
 }
 
 __global__ void reset_counter_kernel(int* counter) { *counter = 0; }
+
+// Plain streaming copy (16 B per lane per step), used to calibrate HBM
+// traffic counters against a kernel of known size.
+__global__ __launch_bounds__(256) void copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
 
 // Bitpack: one wave per (haplotype, 64-site word); the ballot of the 64
 // lanes' bits is the word (bit b of word w = G[h][64 w + b] & 1, tail 0).

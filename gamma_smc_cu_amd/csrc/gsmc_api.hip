@@ -226,10 +226,11 @@ uint32_t descriptor(int steps, int N)
 using namespace gsmc;
 
 struct Tuning {
-    int K = 32;              // checkpoint interval = register window
+    int K = 16;              // checkpoint interval = register window
     int wg = 64;             // workgroup size (64 or 256 lanes)
     int schedule = 0;        // 0 static, 1 persistent queue
-    int xcd_order = 1;       // static schedule: give each XCD a contiguous run of items
+    int xcd_order = 0;       // static schedule: give each XCD a contiguous run of items
+    int occ = 1;             // waves per SIMD the mean-only kernel is built to fit at least
     int64_t min_waves = 8192;
 };
 
@@ -327,7 +328,9 @@ void finish_create(gsmc_context* ctx, const double* positions, double Ne, double
     ctx->Ne = (float)Ne;
     ctx->pos.assign(positions, positions + ctx->S);
     ctx->tabs = get_tables(ctx->device, path, ctx->N, Ne, mu, rho, precision);
-    std::vector<uint32_t> d((size_t)std::max<int64_t>(ctx->S, 1));
+    // 64 zero entries past the last site: kernels load descriptors in
+    // fixed-size runs.
+    std::vector<uint32_t> d((size_t)ctx->S + 64, 0u);
     for (int64_t s = 0; s < ctx->S; ++s) {
         const double delta = s == 0 ? positions[0] + 1.0 : positions[s] - positions[s - 1];
         d[s] = descriptor(round_steps(delta), ctx->N);
@@ -386,7 +389,7 @@ struct Plan {
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-Plan make_plan(const gsmc_context* ctx, const std::vector<JobSpec>& jobs, const Tuning& tu)
+Plan make_plan(const gsmc_context* ctx, const std::vector<JobSpec>& jobs, const Tuning& tu, int wg)
 {
     std::vector<JobDev> jd;
     std::vector<BlockDev> bd;
@@ -428,7 +431,7 @@ Plan make_plan(const gsmc_context* ctx, const std::vector<JobSpec>& jobs, const 
         // Workgroup g runs on XCD g % 8: hand each XCD a contiguous run of
         // workgroup-sized chunks, so the waves sharing an XCD's L2 walk the
         // same blocks (and so the same table layers) at the same time.
-        const int64_t wpw = tu.wg / 64;
+        const int64_t wpw = wg / 64;
         const int64_t G = ((int64_t)items.size() + wpw - 1) / wpw;
         const int64_t per = (G + 7) / 8;
         std::vector<ItemDev> perm((size_t)(per * 8 * wpw), ItemDev{-1, 0});
@@ -453,42 +456,60 @@ Plan make_plan(const gsmc_context* ctx, const std::vector<JobSpec>& jobs, const 
     return pl;
 }
 
-template <int K, bool C, bool Q, int WG>
+template <int K, bool C, bool Q, int WG, int OCC>
 void launch_kernel(const FbArgs& a, int schedule, unsigned grid, hipStream_t st)
 {
     if (schedule == 1)
-        hipLaunchKernelGGL((fb_queue_kernel<K, C, Q, WG>), dim3(grid), dim3(WG), 0, st, a);
+        hipLaunchKernelGGL((fb_queue_kernel<K, C, Q, WG, OCC>), dim3(grid), dim3(WG), 0, st, a);
     else
-        hipLaunchKernelGGL((fb_items_kernel<K, C, Q, WG>), dim3(grid), dim3(WG), 0, st, a);
+        hipLaunchKernelGGL((fb_items_kernel<K, C, Q, WG, OCC>), dim3(grid), dim3(WG), 0, st, a);
 }
 
-template <int K, int WG>
-void dispatch_flags(const FbArgs& a, int flags, int schedule, unsigned grid, hipStream_t st)
+template <int K, bool C, bool Q, int WG, int OCC> const void* queue_fn()
+{
+    return reinterpret_cast<const void*>(&fb_queue_kernel<K, C, Q, WG, OCC>);
+}
+
+// The mean-only kernel (the throughput path) is built for every launch shape
+// the tuner sweeps; the CI / posterior kernels for the checkpoint interval
+// only, with 64-lane workgroups and no occupancy floor.
+template <int K> void dispatch_k(const FbArgs& a, const Tuning& tu, int flags, unsigned grid, hipStream_t st)
 {
     const bool ci = flags & GSMC_CI, post = flags & GSMC_POSTERIOR;
-    if (ci && post) launch_kernel<K, true, true, WG>(a, schedule, grid, st);
-    else if (ci) launch_kernel<K, true, false, WG>(a, schedule, grid, st);
-    else if (post) launch_kernel<K, false, true, WG>(a, schedule, grid, st);
-    else launch_kernel<K, false, false, WG>(a, schedule, grid, st);
+    if (ci && post) return launch_kernel<K, true, true, 64, 1>(a, tu.schedule, grid, st);
+    if (ci) return launch_kernel<K, true, false, 64, 1>(a, tu.schedule, grid, st);
+    if (post) return launch_kernel<K, false, true, 64, 1>(a, tu.schedule, grid, st);
+    constexpr int kOcc = K == 8 ? 6 : 1;  // the floor is built for K = 8 only
+    if (tu.wg == 256) {
+        if (tu.occ >= 6) return launch_kernel<K, false, false, 256, kOcc>(a, tu.schedule, grid, st);
+        return launch_kernel<K, false, false, 256, 1>(a, tu.schedule, grid, st);
+    }
+    if (tu.occ >= 6) return launch_kernel<K, false, false, 64, kOcc>(a, tu.schedule, grid, st);
+    return launch_kernel<K, false, false, 64, 1>(a, tu.schedule, grid, st);
 }
+
+template <int K> const void* queue_kernel_of(const Tuning& tu, int flags)
+{
+    const bool ci = flags & GSMC_CI, post = flags & GSMC_POSTERIOR;
+    if (ci && post) return queue_fn<K, true, true, 64, 1>();
+    if (ci) return queue_fn<K, true, false, 64, 1>();
+    if (post) return queue_fn<K, false, true, 64, 1>();
+    constexpr int kOcc = K == 8 ? 6 : 1;
+    if (tu.wg == 256) return tu.occ >= 6 ? queue_fn<K, false, false, 256, kOcc>() : queue_fn<K, false, false, 256, 1>();
+    return tu.occ >= 6 ? queue_fn<K, false, false, 64, kOcc>() : queue_fn<K, false, false, 64, 1>();
+}
+
+// Workgroup size a launch uses under tuning tu and flags.
+int launch_wg(const Tuning& tu, int flags) { return (flags & (GSMC_CI | GSMC_POSTERIOR)) ? 64 : tu.wg; }
 
 // Waves of the decode kernel one device holds at once (persistent grid size).
 int64_t resident_waves(const gsmc_context* ctx, const Tuning& tu, int flags)
 {
     int per_cu = 0;
-    const bool ci = flags & GSMC_CI, post = flags & GSMC_POSTERIOR;
-    const void* fn = nullptr;
-#define GSMC_FN(KK, WW)                                                                        \
-    fn = ci ? (post ? (const void*)fb_queue_kernel<KK, true, true, WW> : (const void*)fb_queue_kernel<KK, true, false, WW>) \
-            : (post ? (const void*)fb_queue_kernel<KK, false, true, WW> : (const void*)fb_queue_kernel<KK, false, false, WW>);
-    if (tu.K == 16) {
-        if (tu.wg == 256) { GSMC_FN(16, 256) } else { GSMC_FN(16, 64) }
-    } else {
-        if (tu.wg == 256) { GSMC_FN(32, 256) } else { GSMC_FN(32, 64) }
-    }
-#undef GSMC_FN
-    HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, tu.wg, 0));
-    return (int64_t)std::max(per_cu, 1) * ctx->cus * (tu.wg / 64);
+    const void* fn = tu.K == 8 ? queue_kernel_of<8>(tu, flags) : queue_kernel_of<16>(tu, flags);
+    const int wg = launch_wg(tu, flags);
+    HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, wg, 0));
+    return (int64_t)std::max(per_cu, 1) * ctx->cus * (wg / 64);
 }
 
 // Queue one launch of `jobs` on `stream` using workspace b.
@@ -496,7 +517,7 @@ void launch_plan(gsmc_batch* b, const std::vector<JobSpec>& jobs, int flags, hip
 {
     gsmc_context* ctx = b->ctx;
     const Tuning tu = ctx->tune;
-    Plan pl = make_plan(ctx, jobs, tu);
+    Plan pl = make_plan(ctx, jobs, tu, launch_wg(tu, flags));
     if (pl.n_items == 0) return;
     const bool same = b->pending && stream == b->last_stream && pl.bytes == b->last;
     if (!same) {
@@ -524,7 +545,7 @@ void launch_plan(gsmc_batch* b, const std::vector<JobSpec>& jobs, int flags, hip
     a.ck = b->ck.as<float2>();
     a.t = ctx->tabs->view;
     a.unscale = 2.0f * ctx->Ne;
-    const int64_t wpw = tu.wg / 64;
+    const int64_t wpw = launch_wg(tu, flags) / 64;
     unsigned grid;
     if (tu.schedule == 1) {
         hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, stream, a.queue);
@@ -532,13 +553,8 @@ void launch_plan(gsmc_batch* b, const std::vector<JobSpec>& jobs, int flags, hip
     } else {
         grid = (unsigned)((pl.n_items + wpw - 1) / wpw);
     }
-    if (tu.K == 16) {
-        if (tu.wg == 256) dispatch_flags<16, 256>(a, flags, tu.schedule, grid, stream);
-        else dispatch_flags<16, 64>(a, flags, tu.schedule, grid, stream);
-    } else {
-        if (tu.wg == 256) dispatch_flags<32, 256>(a, flags, tu.schedule, grid, stream);
-        else dispatch_flags<32, 64>(a, flags, tu.schedule, grid, stream);
-    }
+    if (tu.K == 8) dispatch_k<8>(a, tu, flags, grid, stream);
+    else dispatch_k<16>(a, tu, flags, grid, stream);
     HIP_OK(hipGetLastError());
     HIP_OK(hipEventRecord(b->done, stream));
     b->pending = true;
@@ -867,12 +883,13 @@ int gsmc_context_info(const gsmc_context* ctx, int64_t* n, int64_t* S, int* devi
     });
 }
 
-int gsmc_context_set_tuning(gsmc_context* ctx, int k, int workgroup, int schedule, int xcd_order,
+int gsmc_context_set_tuning(gsmc_context* ctx, int k, int workgroup, int schedule, int xcd_order, int occupancy,
                             int64_t min_waves)
 {
     return guarded([&] {
+        require(occupancy == 1 || (occupancy == 6 && k == 8), "occupancy must be 1 (no floor), or 6 with checkpoint_k 8");
         require(ctx != nullptr, "context is NULL");
-        require(k == 16 || k == 32, "checkpoint_k must be 16 or 32");
+        require(k == 8 || k == 16, "checkpoint_k must be 8 or 16");
         require(workgroup == 64 || workgroup == 256, "workgroup must be 64 or 256");
         require(schedule == 0 || schedule == 1, "schedule must be 0 (static) or 1 (persistent)");
         require(min_waves >= 1, "min_waves_per_launch must be positive");
@@ -881,6 +898,7 @@ int gsmc_context_set_tuning(gsmc_context* ctx, int k, int workgroup, int schedul
         ctx->tune.wg = workgroup;
         ctx->tune.schedule = schedule;
         ctx->tune.xcd_order = xcd_order ? 1 : 0;
+        ctx->tune.occ = occupancy;
         ctx->tune.min_waves = min_waves;
     });
 }
@@ -1070,6 +1088,19 @@ int gsmc_checkpoint_bytes(gsmc_context* ctx, int64_t n_sites, int64_t core, int6
         for (const Block& b : make_blocks(n_sites, core > 0 ? core : std::max<int64_t>(n_sites, 1), core > 0 ? flank : 0))
             windows += (uint64_t)((b.ce - b.cs + ctx->tune.K - 1) / ctx->tune.K);
         *bytes = windows * (uint64_t)tiles * 64 * sizeof(float2);
+    });
+}
+
+int gsmc_device_copy(void* dst, const void* src, uint64_t bytes, void* stream)
+{
+    return guarded([&] {
+        require(bytes % 16 == 0, "copy size must be a multiple of 16 bytes");
+        const int64_t n = (int64_t)(bytes / 16);
+        if (n == 0) return;
+        const unsigned grid = (unsigned)std::min<int64_t>((n + 255) / 256, 65536);
+        hipLaunchKernelGGL(copy_kernel, dim3(grid), dim3(256), 0, static_cast<hipStream_t>(stream),
+                           static_cast<const uint4*>(src), static_cast<uint4*>(dst), n);
+        HIP_OK(hipGetLastError());
     });
 }
 

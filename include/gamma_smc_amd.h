@@ -115,14 +115,17 @@ void gsmc_context_destroy(gsmc_context* ctx);
 int gsmc_context_info(const gsmc_context* ctx, int64_t* n_haps, int64_t* n_sites, int* device_id, float* Ne);
 
 /* Decoder tuning; results do not depend on any of it. checkpoint_k: sites
- * per checkpoint window, 16 or 32 (default 32); workgroup: 64 or 256 lanes
+ * per checkpoint window, 8 or 16 (default 16); workgroup: 64 or 256 lanes
  * (default 64); schedule: 0 static grid of work items, 1 persistent waves
  * draining a device queue (default 0); xcd_order: static grid ordered so
- * each XCD walks a contiguous run of items (default 1); min_waves: work
- * items the host-output path puts in one launch at least (default 8192).
+ * each XCD walks a contiguous run of items (default 0); occupancy: 1 (no
+ * floor) or 6 waves per SIMD the mean-only kernel is compiled to fit
+ * (default 1); min_waves: work items the host-output path puts in one
+ * launch at least (default 8192). Workgroup and occupancy apply to the
+ * mean-only kernel; CI / posterior launches use 64 lanes and no floor.
  * No reference counterpart (the reference has fixed 256-thread launches). */
 int gsmc_context_set_tuning(gsmc_context* ctx, int checkpoint_k, int workgroup, int schedule, int xcd_order,
-                            int64_t min_waves_per_launch);
+                            int occupancy, int64_t min_waves_per_launch);
 
 /* Full-sequence forward-backward of n_pairs pairs (pair_i[p], pair_j[p]) into
  * host arrays laid out site-major [n_sites][n_pairs]: mean always; lower and
@@ -195,6 +198,9 @@ int gsmc_event_create(void** event);
 int gsmc_event_destroy(void* event);
 int gsmc_event_record(void* event, void* stream);
 int gsmc_event_elapsed_ms(void* start, void* stop, float* ms);
+/* Streaming device-to-device copy kernel (bytes a multiple of 16), queued
+ * on `stream`; a kernel of known traffic for counter calibration. */
+int gsmc_device_copy(void* dst, const void* src, uint64_t bytes, void* stream);
 
 #ifdef __cplusplus
 }
