@@ -112,8 +112,7 @@ def main():
     ap.add_argument("--tune", default="auto", choices=("auto", "none"),
                     help="auto: sweep launch shapes first and keep the fastest (results are identical)")
     ap.add_argument("--checkpoint-k", type=int, default=16)
-    ap.add_argument("--workgroup", type=int, default=64)
-    ap.add_argument("--occupancy", type=int, default=1)
+    ap.add_argument("--pairs-per-lane", type=int, default=1)
     ap.add_argument("--schedule", default="static", choices=("static", "persistent"))
     ap.add_argument("--xcd-order", type=int, default=0)
     ap.add_argument("--cpu-pairs", type=int, default=8000)
@@ -165,8 +164,8 @@ def main():
         stream.synchronize()
         return e0.elapsed_ms(e1)
 
-    tuning = dict(checkpoint_k=args.checkpoint_k, workgroup=args.workgroup, occupancy=args.occupancy,
-                  schedule=args.schedule, xcd_order=bool(args.xcd_order))
+    tuning = dict(checkpoint_k=args.checkpoint_k, pairs_per_lane=args.pairs_per_lane, schedule=args.schedule,
+                  xcd_order=bool(args.xcd_order))
     sweep = None
     if args.tuning_json:
         with open(args.tuning_json) as fh:
@@ -238,7 +237,8 @@ def main():
                    "tuning": tuning, "mode": "mean_only", "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "fb_items_kernel" if tuning["schedule"] == "static" else "fb_queue_kernel",
+                     "kernel": ("fb_fwd_items + fb_bwd_items" if tuning["schedule"] == "static"
+                                else "fb_fwd_queue + fb_bwd_queue"),
                      "algorithmic_bytes_per_launch": B},
         "cpu_baseline": cpu,
         "checks": checks,

@@ -75,7 +75,7 @@ def _declare(L):
                                        ctypes.c_char_p, _i32, _i32, ctypes.POINTER(_vp)],
         "gsmc_context_info": [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(ctypes.c_int),
                               ctypes.POINTER(ctypes.c_float)],
-        "gsmc_context_set_tuning": [_vp, _i32, _i32, _i32, _i32, _i32, _i64],
+        "gsmc_context_set_tuning": [_vp, _i32, _i32, _i32, _i32, _i64],
         "gsmc_run_fb": [_vp, _vp, _vp, _i64, _i32, _vp, _vp, _vp, _vp, _vp],
         "gsmc_block_windows": [_i64, _i64, _i64, _vp, _i64, ctypes.POINTER(_i64)],
         "gsmc_run_fb_blockwise": [_vp, _vp, _vp, _i64, _i64, _i64, _i64, _i32, _i32, _vp, _vp, _vp, _vp, _vp],
@@ -281,12 +281,12 @@ class FlowContext:
         except Exception:
             pass
 
-    def set_tuning(self, checkpoint_k=16, workgroup=64, schedule="static", xcd_order=False, occupancy=1,
+    def set_tuning(self, checkpoint_k=16, pairs_per_lane=1, schedule="static", xcd_order=False,
                    min_waves_per_launch=8192):
         """Launch shape (results are identical under every setting)."""
         sched = {"static": 0, "persistent": 1}[schedule]
-        check(lib().gsmc_context_set_tuning(self._h, int(checkpoint_k), int(workgroup), sched, int(bool(xcd_order)),
-                                            int(occupancy), int(min_waves_per_launch)))
+        check(lib().gsmc_context_set_tuning(self._h, int(checkpoint_k), int(pairs_per_lane), sched,
+                                            int(bool(xcd_order)), int(min_waves_per_launch)))
 
     def checkpoint_bytes(self, n_sites, core_block_sites, flank_sites, n_pairs):
         out = ctypes.c_uint64()

@@ -1,15 +1,15 @@
 """Launch-shape autotuner for the decode kernel.
 
 Every candidate computes bit-identical results (the launch shape decides
-only which wave decodes which 64-pair tile of which block window, and how
+only which wave decodes which pair tile of which block window, and how
 many sites a checkpoint window spans), so the tuner is free to pick the
 fastest. Candidates sweep:
 
 * ``checkpoint_k`` -- sites per checkpoint window, which is also the length
   of the forward window recomputed into registers (8 or 16);
-* ``workgroup`` -- 64 or 256 lanes (1 or 4 pair tiles per workgroup);
-* ``occupancy`` -- no floor, or (K = 8) a kernel compiled to fit 6 waves per
-  SIMD (fewer registers for the two interleaved lookup chains, more waves);
+* ``pairs_per_lane`` -- 1 or 2 pairs per lane, i.e. 64 or 128 pairs per
+  64-lane workgroup (two pairs give every phase two independent lookup
+  chains per lane, at the cost of registers);
 * ``schedule`` -- a static grid of work items, ordered per XCD or not, or
   persistent waves draining a device queue.
 
@@ -24,10 +24,9 @@ import itertools
 import time
 
 CANDIDATES = [
-    dict(checkpoint_k=k, workgroup=wg, occupancy=occ, schedule=sch, xcd_order=xcd)
-    for k, occ, wg, (sch, xcd) in itertools.product(
-        (8, 16), (1, 6), (64, 256), (("static", False), ("static", True), ("persistent", False)))
-    if occ == 1 or k == 8  # the occupancy floor is built for K = 8 only
+    dict(checkpoint_k=k, pairs_per_lane=np_, schedule=sch, xcd_order=xcd)
+    for k, np_, (sch, xcd) in itertools.product(
+        (8, 16), (1, 2), (("static", False), ("static", True), ("persistent", False)))
 ]
 
 

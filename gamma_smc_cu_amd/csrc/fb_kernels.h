@@ -94,7 +94,7 @@ struct BlockDev {
     int64_t ck_off;  // float2 offset of this block's checkpoints (tile-major)
 };
 
-// A work item: one 64-pair tile of one block window.
+// A work item: one tile (64 pairs per lane slot) of one block window.
 struct ItemDev {
     int32_t block;
     int32_t tile;
@@ -110,24 +110,32 @@ struct FbArgs {
     const ItemDev* items;
     int64_t n_items;
     int* queue;                // persistent mode: next item counter (starts at 0)
-    float2* ck;                // checkpoints: per item, [nwin][64]
+    float2* ck;                // checkpoints: per item, [nwin][64 NP]
+    float2* yb;                // backward state at each item's core end: per item, [64 NP]
+    int* queue2;               // persistent mode: item counter of the second kernel
     DevTables t;
     float unscale;             // 2 Ne
 };
 
-// Bilinear lookup of one table slab at (m, c); updates (m, c).
-// het_off selects the hom (0) or het (kSlabBytes) slab of a layer.
-__device__ __forceinline__ void lookup(const char* __restrict__ layer, uint32_t het_off, float& m, float& c)
+// Bilinear lookup of one table slab at (m, c); updates (m, c). het (0 or
+// 1) selects the hom or het slab of the layer.
+//
+// Same values as gamma_smc_flow.cu:408-430 in fewer instructions: the clamp
+// is one med3 (identical for any non-NaN index), the cell corner is
+// floor(index) capped at the last cell (identical to truncating a
+// non-negative index and decrementing 50 -> 49 / 49 -> 48), and the byte
+// offset of the corner is formed in float, exact for these small integers.
+__device__ __forceinline__ void lookup(const char* __restrict__ layer, uint32_t het, float& m, float& c)
 {
     float fm = (m - -5.0f) * kMeanStepInv;
     float fc = (c - -2.0f) * kCvStepInv;
-    fm = fmaxf(0.0f, fminf(fm, 50.0f));
-    fc = fmaxf(0.0f, fminf(fc, 49.0f));
-    const int m0 = min((int)fm, 49);
-    const int c0 = min((int)fc, 48);
-    const float wm = fm - (float)m0;
-    const float wc = fc - (float)c0;
-    const uint32_t off = het_off + __umul24((uint32_t)m0, 400u) + (uint32_t)c0 * 8u;
+    fm = __builtin_amdgcn_fmed3f(fm, 0.0f, 50.0f);
+    fc = __builtin_amdgcn_fmed3f(fc, 0.0f, 49.0f);
+    const float lm = fminf(floorf(fm), 49.0f);
+    const float lc = fminf(floorf(fc), 48.0f);
+    const float wm = fm - lm;
+    const float wc = fc - lc;
+    const uint32_t off = __umul24(het, kSlabBytes) + (uint32_t)fmaf(lm, 400.0f, lc * 8.0f);
     const float2 v00 = *reinterpret_cast<const float2*>(layer + off);
     const float2 v01 = *reinterpret_cast<const float2*>(layer + off + 8);
     const float2 v10 = *reinterpret_cast<const float2*>(layer + off + 400);
@@ -137,19 +145,24 @@ __device__ __forceinline__ void lookup(const char* __restrict__ layer, uint32_t 
     c = fmaf(wm, fmaf(wc, v11.y, omwc * v10.y), omwm * fmaf(wc, v01.y, omwc * v00.y));
 }
 
+__device__ __forceinline__ const char* layer_of(const char* base, uint32_t d)
+{
+    return base + (size_t)(d & kDescLayerMask) * kLayerBytes;
+}
+
 // Forward segment: full N-bp hom stretches, then the final table of
 // (remaining bp, observed site).
-__device__ __forceinline__ void fwd_segment(const DevTables& t, uint32_t d, uint32_t het_off, float& m, float& c)
+__device__ __forceinline__ void fwd_segment(const DevTables& t, uint32_t d, uint32_t het, float& m, float& c)
 {
     for (uint32_t q = d >> kDescLayerBits; q != 0; --q) lookup(t.stretch, 0, m, c);
-    lookup(t.fwd + (size_t)(d & kDescLayerMask) * kLayerBytes, het_off, m, c);
+    lookup(layer_of(t.fwd, d), het, m, c);
 }
 
 // Backward segment: observed site with the remaining bp first, then the
 // full N-bp stretches.
-__device__ __forceinline__ void bwd_segment(const DevTables& t, uint32_t d, uint32_t het_off, float& m, float& c)
+__device__ __forceinline__ void bwd_segment(const DevTables& t, uint32_t d, uint32_t het, float& m, float& c)
 {
-    lookup(t.bwd + (size_t)(d & kDescLayerMask) * kLayerBytes, het_off, m, c);
+    lookup(layer_of(t.bwd, d), het, m, c);
     for (uint32_t q = d >> kDescLayerBits; q != 0; --q) lookup(t.stretch, 0, m, c);
 }
 
@@ -173,6 +186,13 @@ __device__ __forceinline__ uint64_t het_mask(const uint64_t* __restrict__ ri, co
     return (x0 >> sh) | (x1 << (64 - sh));
 }
 
+// Bit k of a het mask, k a compile-time constant in the unrolled loops.
+__device__ __forceinline__ uint32_t het_bit(uint64_t bits, int k)
+{
+    const uint32_t word = k < 32 ? (uint32_t)bits : (uint32_t)(bits >> 32);
+    return (word >> (k & 31)) & 1u;
+}
+
 // K consecutive site descriptors from s (the array is padded by 64 entries
 // past the last site), through scalar loads.
 template <int K>
@@ -183,9 +203,15 @@ __device__ __forceinline__ void load_descs(const uint32_t* desc, int64_t s, uint
     for (int k = 0; k < K; ++k) d[k] = p[k];
 }
 
-__device__ __forceinline__ uint32_t het_off_of(uint64_t bits, int k)
+// True when none of the K descriptors carries a full-stretch chunk (gap
+// beyond the table depth): the window's lookups are then one table layer
+// each and need no loop.
+template <int K> __device__ __forceinline__ bool no_long_gaps(const uint32_t (&d)[K])
 {
-    return ((uint32_t)(bits >> k) & 1u) * kSlabBytes;
+    uint32_t any = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) any |= d[k];
+    return (any >> kDescLayerBits) == 0;
 }
 
 // Streaming store that bypasses cache retention (outputs are never re-read).
@@ -194,7 +220,6 @@ __device__ __forceinline__ void store_stream(float* row, uint32_t col, float v)
     __builtin_nontemporal_store(v, global_at<float>(row, col));
 }
 
-template <bool CI, bool POST>
 __device__ __forceinline__ float combine_mean(float fm, float fc, float ym, float yc, float unscale, float& as,
                                               float& bs)
 {
@@ -207,7 +232,7 @@ __device__ __forceinline__ float combine_mean(float fm, float fc, float ym, floa
     return (as / fmaxf(bs, 1e-10f)) * unscale;
 }
 
-// The CI / posterior outputs of one site, stored at once.
+// The CI / posterior outputs of one site.
 template <bool CI, bool POST>
 __device__ __forceinline__ void store_extras(const JobDev& j, size_t row, uint32_t col, float mg, float as, float bs)
 {
@@ -228,152 +253,254 @@ __device__ __forceinline__ void store_extras(const JobDev& j, size_t row, uint32
     }
 }
 
-// True when none of the K descriptors carries a full-stretch chunk (gap
-// beyond the table depth): the window's lookups are then one table layer
-// each and need no loop.
-template <int K> __device__ __forceinline__ bool no_long_gaps(const uint32_t (&d)[K])
+// Per-lane view of a work item: NP pairs per lane (lane l of the wave
+// holds pairs tile * 64 NP + q * 64 + l, q < NP), their haplotype rows,
+// output columns and checkpoint columns.
+template <int NP> struct Lanes {
+    const uint64_t* ri[NP];
+    const uint64_t* rj[NP];
+    uint32_t col[NP];   // byte offset of the pair in an output row
+    bool live[NP];      // the pair exists (idle lanes shadow the job's last pair)
+    uint32_t ck_col;    // byte offset of the lane in a checkpoint row of 64 NP float2
+    float2* ck;         // checkpoint rows of the item: window w at ck + w * 64 NP
+};
+
+template <int K, int NP> struct Window {
+    uint32_t d[K];     // site descriptors
+    uint64_t bits[NP]; // het masks
+};
+
+template <int K, int NP>
+__device__ __forceinline__ void load_window(const FbArgs& a, const Lanes<NP>& L, int64_t ws, Window<K, NP>& win)
 {
-    uint32_t any = 0;
+    load_descs<K>(a.desc, ws, win.d);
 #pragma unroll
-    for (int k = 0; k < K; ++k) any |= d[k];
-    return (any >> kDescLayerBits) == 0;
+    for (int q = 0; q < NP; ++q) win.bits[q] = het_mask(L.ri[q], L.rj[q], ws, a.W);
 }
 
-__device__ __forceinline__ const char* layer_of(const char* base, uint32_t d)
+template <int NP>
+__device__ __forceinline__ float2* ck_at(const Lanes<NP>& L, int64_t w, int q)
 {
-    return base + (size_t)(d & kDescLayerMask) * kLayerBytes;
+    return global_at<float2>(L.ck + (size_t)w * 64 * NP, L.ck_col + (uint32_t)q * 512u);
 }
 
 // Phase-3 step over one checkpoint window, software-pipelined: the
 // backward walk of window w (cnt sites at ws, forward states in F) runs in
-// the same loop as the forward recompute of window w - 1 (K sites at
-// ws - K) when has_next. The two chains are independent, so each wave
-// keeps two lookups in flight. The recomputed states reuse the slots the
-// backward walk frees, which flips the slot order every window:
-// orientation A holds site ws + k in slot k, orientation B in slot K-1-k.
-// Window outputs are buffered and stored after the loop, so no lookup of
-// the window waits on an output store. FAST (a full window and a full next
-// window, no long gaps in either) has no branch inside the loop, so the
-// compiler can interleave the two chains' instructions.
-template <int K, bool CI, bool POST, bool B, bool FAST>
-__device__ __forceinline__ void window_step(const FbArgs& a, const JobDev& j, int64_t ws, int64_t cnt, bool has_next,
-                                            const uint32_t (&dc)[K], uint64_t bc, const uint32_t (&dn)[K],
-                                            uint64_t bn, float m, float c, uint32_t col, bool live, float (&fm)[K],
-                                            float (&fc)[K], float& ym, float& yc)
+// the same loop as the forward recompute of window w - 1 (K sites) when
+// has_next. The two chains are independent, so each pair keeps two lookups
+// in flight. The recomputed states reuse the slots the backward walk frees,
+// which flips the slot order every window: orientation A holds site ws + k
+// in slot k, orientation B in slot K-1-k. Window outputs are buffered and
+// stored after the loop, so no lookup of the window waits on an output
+// store (vector-memory counters are shared by loads and stores). FAST (a
+// full window and a full next window, no long gaps in either) has no branch
+// inside the loop, so the compiler can interleave all chains.
+template <int K, int NP, bool CI, bool POST, bool B, bool FAST>
+__device__ __forceinline__ void window_step(const FbArgs& a, const JobDev& j, const Lanes<NP>& L, int64_t ws,
+                                            int64_t cnt, bool has_next, const Window<K, NP>& cur,
+                                            const Window<K, NP>& nxt, float (&m)[NP], float (&c)[NP],
+                                            float (&fm)[NP][K], float (&fc)[NP][K], float (&ym)[NP],
+                                            float (&yc)[NP])
 {
-    float out[K], oa[K], ob[K];
+    float out[NP][K], oa[NP][K], ob[NP][K];
 #pragma unroll
     for (int i = 0; i < K; ++i) {
         const int k = K - 1 - i;
         const int slot = B ? i : k;  // current window: site ws + k
         if (FAST || k < cnt) {
-            out[k] = combine_mean<CI, POST>(fm[slot], fc[slot], ym, yc, a.unscale, oa[k], ob[k]);
-            if (FAST)
-                lookup(layer_of(a.t.bwd, dc[k]), het_off_of(bc, k), ym, yc);
-            else
-                bwd_segment(a.t, dc[k], het_off_of(bc, k), ym, yc);
+#pragma unroll
+            for (int q = 0; q < NP; ++q) {
+                out[q][k] = combine_mean(fm[q][slot], fc[q][slot], ym[q], yc[q], a.unscale, oa[q][k], ob[q][k]);
+                if (FAST)
+                    lookup(layer_of(a.t.bwd, cur.d[k]), het_bit(cur.bits[q], k), ym[q], yc[q]);
+                else
+                    bwd_segment(a.t, cur.d[k], het_bit(cur.bits[q], k), ym[q], yc[q]);
+            }
         }
         if (FAST || has_next) {
-            if (FAST)
-                lookup(layer_of(a.t.fwd, dn[i]), het_off_of(bn, i), m, c);
-            else
-                fwd_segment(a.t, dn[i], het_off_of(bn, i), m, c);
-            fm[slot] = m;  // next window: site ws - K + i, in the slot just freed
-            fc[slot] = c;
+#pragma unroll
+            for (int q = 0; q < NP; ++q) {
+                if (FAST)
+                    lookup(layer_of(a.t.fwd, nxt.d[i]), het_bit(nxt.bits[q], i), m[q], c[q]);
+                else
+                    fwd_segment(a.t, nxt.d[i], het_bit(nxt.bits[q], i), m[q], c[q]);
+                fm[q][slot] = m[q];  // next window: site ws - K + i, in the slot just freed
+                fc[q][slot] = c[q];
+            }
         }
+        // Keep each site's work in its own scheduling region: the two chains
+        // still interleave inside it, but nothing is hoisted from later
+        // sites, which would cost registers and so waves.
+        if (FAST) __builtin_amdgcn_sched_barrier(0);
     }
-    if (live) {
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+        if (!L.live[q]) continue;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             if (FAST || k < cnt) {
                 const size_t row = (size_t)(ws + k - j.row0) * (size_t)j.P;
-                store_stream(j.mean + row, col, out[k]);
-                if (CI || POST) store_extras<CI, POST>(j, row, col, out[k], oa[k], ob[k]);
+                store_stream(j.mean + row, L.col[q], out[q][k]);
+                if (CI || POST) store_extras<CI, POST>(j, row, L.col[q], out[q][k], oa[q][k], ob[q][k]);
             }
         }
     }
 }
 
-// Decode one work item (64 pairs x one block window) with the calling wave.
-template <int K, bool CI, bool POST>
-__device__ __forceinline__ void decode_item(const FbArgs& a, int item_in, int lane)
+// The item's metadata and per-lane view, shared by both halves of a decode.
+template <int NP> struct ItemView {
+    BlockDev b;
+    JobDev j;
+    Lanes<NP> L;
+    bool valid;
+};
+
+template <int NP>
+__device__ __forceinline__ ItemView<NP> open_item(const FbArgs& a, int item_in, int lane)
 {
+    ItemView<NP> v;
     // The item is the same for the whole wave; saying so keeps every piece
     // of block and job metadata, and each site's descriptor, in scalar
     // registers and scalar loads.
     const int item = __builtin_amdgcn_readfirstlane(item_in);
     const ItemDev it = load_const(a.items + item);
-    if (it.block < 0) return;  // padding of the XCD-ordered schedule
-    const BlockDev b = load_const(a.blocks + it.block);
-    const JobDev j = load_const(a.jobs + b.job);
-    const int64_t p = (int64_t)it.tile * 64 + lane;
-    const bool live = p < j.P;
-    const int64_t pp = live ? p : j.P - 1;  // idle lanes shadow the job's last pair
-    const uint32_t col = (uint32_t)p * 4u;
-    const int64_t cs = b.cs, ce = b.ce, ps = b.ps, pe = b.pe;
-    // Pair ids from device memory are clamped to the cohort, so a bad id can
-    // only produce a wrong row, never an out-of-bounds read; the host entry
-    // points validate ids before they reach the device.
-    const int64_t hi = min(max((int64_t)j.pair_i[pp], (int64_t)0), a.n_haps - 1);
-    const int64_t hj = min(max((int64_t)j.pair_j[pp], (int64_t)0), a.n_haps - 1);
-    const uint64_t* ri = a.packed + (size_t)hi * (size_t)a.W;
-    const uint64_t* rj = a.packed + (size_t)hj * (size_t)a.W;
-    // Every lane, idle ones included, owns a checkpoint column: window w of
-    // this item lives at ck_tile[w * 64 + lane].
-    float2* ck_tile = a.ck + (size_t)b.ck_off + (size_t)it.tile * (size_t)b.nwin * 64;
-    const uint32_t ck_lane = (uint32_t)lane * 8u;
+    v.valid = it.block >= 0;  // padding of the XCD-ordered schedule
+    if (!v.valid) return v;
+    v.b = load_const(a.blocks + it.block);
+    v.j = load_const(a.jobs + v.b.job);
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+        const int64_t p = (int64_t)it.tile * 64 * NP + q * 64 + lane;
+        v.L.live[q] = p < v.j.P;
+        const int64_t pp = v.L.live[q] ? p : v.j.P - 1;
+        v.L.col[q] = (uint32_t)p * 4u;
+        // Pair ids from device memory are clamped to the cohort, so a bad id
+        // can only produce a wrong row, never an out-of-bounds read; the host
+        // entry points validate ids before they reach the device.
+        const int64_t hi = min(max((int64_t)v.j.pair_i[pp], (int64_t)0), a.n_haps - 1);
+        const int64_t hj = min(max((int64_t)v.j.pair_j[pp], (int64_t)0), a.n_haps - 1);
+        v.L.ri[q] = a.packed + (size_t)hi * (size_t)a.W;
+        v.L.rj[q] = a.packed + (size_t)hj * (size_t)a.W;
+    }
+    // Every lane, idle ones included, owns its checkpoint columns.
+    v.L.ck = a.ck + (size_t)v.b.ck_off + (size_t)it.tile * (size_t)v.b.nwin * 64 * NP;
+    v.L.ck_col = (uint32_t)lane * 8u;
+    return v;
+}
 
-    // Phase 1: forward over [ps, ce), checkpoints at the core windows.
-    float m = 0.0f, c = 0.0f;
+template <int NP> __device__ __forceinline__ float2* yb_at(const FbArgs& a, int item, int lane, int q)
+{
+    return global_at<float2>(a.yb + (size_t)item * 64 * NP, (uint32_t)(q * 64 + lane) * 8u);
+}
+
+// First half of an item (64 NP pairs x one block window): the forward walk
+// over [ps, ce) storing the state at the start of every core window, and
+// the backward walk over the right flank [ce, pe), whose final state is
+// the backward state entering the core.
+template <int K, int NP>
+__device__ __forceinline__ void forward_item(const FbArgs& a, int item, int lane)
+{
+    const ItemView<NP> v = open_item<NP>(a, item, lane);
+    if (!v.valid) return;
+    const BlockDev& b = v.b;
+    const Lanes<NP>& L = v.L;
+    const int64_t cs = b.cs, ce = b.ce, ps = b.ps, pe = b.pe;
+
+    float m[NP], c[NP];
+#pragma unroll
+    for (int q = 0; q < NP; ++q) m[q] = c[q] = 0.0f;
     for (int64_t s = ps; s < ce;) {
         const int64_t e = (s < cs) ? min(cs, s + K) : min(ce, s + K);
-        if (s >= cs) *global_at<float2>(ck_tile + (size_t)((s - cs) / K) * 64, ck_lane) = make_float2(m, c);
-        const uint64_t bits = het_mask(ri, rj, s, a.W);
-        uint32_t d[K];
-        load_descs<K>(a.desc, s, d);
-        if (s == ps) d[0] = b.sd_fwd;
+        if (s >= cs) {
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            if (s + k < e) fwd_segment(a.t, d[k], het_off_of(bits, k), m, c);
+            for (int q = 0; q < NP; ++q) *ck_at(L, (s - cs) / K, q) = make_float2(m[q], c[q]);
+        }
+        Window<K, NP> win;
+        load_window(a, L, s, win);
+        if (s == ps) win.d[0] = b.sd_fwd;
+        if (e - s == K && no_long_gaps<K>(win.d)) {
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+#pragma unroll
+                for (int q = 0; q < NP; ++q) lookup(layer_of(a.t.fwd, win.d[k]), het_bit(win.bits[q], k), m[q], c[q]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                if (s + k < e)
+#pragma unroll
+                    for (int q = 0; q < NP; ++q) fwd_segment(a.t, win.d[k], het_bit(win.bits[q], k), m[q], c[q]);
         }
         s = e;
     }
 
-    // Phase 2: backward over the right flank [ce, pe).
-    float ym = 0.0f, yc = 0.0f;
+    float ym[NP], yc[NP];
+#pragma unroll
+    for (int q = 0; q < NP; ++q) ym[q] = yc[q] = 0.0f;
     for (int64_t e = pe; e > ce;) {
         const int64_t b0 = max(ce, e - K);
-        const uint64_t bits = het_mask(ri, rj, b0, a.W);
-        uint32_t d[K];
-        load_descs<K>(a.desc, b0, d);
+        Window<K, NP> win;
+        load_window(a, L, b0, win);
+        if (e - b0 == K && no_long_gaps<K>(win.d)) {
 #pragma unroll
-        for (int k = K - 1; k >= 0; --k) {
-            if (b0 + k < e) bwd_segment(a.t, d[k], het_off_of(bits, k), ym, yc);
+            for (int k = K - 1; k >= 0; --k)
+#pragma unroll
+                for (int q = 0; q < NP; ++q) lookup(layer_of(a.t.bwd, win.d[k]), het_bit(win.bits[q], k), ym[q], yc[q]);
+        } else {
+#pragma unroll
+            for (int k = K - 1; k >= 0; --k)
+                if (b0 + k < e)
+#pragma unroll
+                    for (int q = 0; q < NP; ++q) bwd_segment(a.t, win.d[k], het_bit(win.bits[q], k), ym[q], yc[q]);
         }
         e = b0;
     }
+#pragma unroll
+    for (int q = 0; q < NP; ++q) *yb_at<NP>(a, __builtin_amdgcn_readfirstlane(item), lane, q) = make_float2(ym[q], yc[q]);
+}
 
-    // Phase 3: core windows, last to first. The last window (possibly
-    // partial) is recomputed alone into orientation A; every later
-    // recompute rides along the previous window's backward walk.
-    float fm[K], fc[K];
+// Second half of an item: the core windows, last to first. The last window
+// (possibly partial) is recomputed alone into orientation A; every later
+// recompute rides along the previous window's backward walk.
+template <int K, int NP, bool CI, bool POST>
+__device__ __forceinline__ void backward_item(const FbArgs& a, int item, int lane)
+{
+    const ItemView<NP> v = open_item<NP>(a, item, lane);
+    if (!v.valid) return;
+    const BlockDev& b = v.b;
+    const JobDev& j = v.j;
+    const Lanes<NP>& L = v.L;
+    const int64_t cs = b.cs, ce = b.ce, ps = b.ps;
+
+    float m[NP], c[NP], ym[NP], yc[NP];
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+        const float2 y = *yb_at<NP>(a, __builtin_amdgcn_readfirstlane(item), lane, q);
+        ym[q] = y.x;
+        yc[q] = y.y;
+    }
+    float fm[NP][K], fc[NP][K];
     int64_t w = (int64_t)b.nwin - 1;
     int64_t cnt = ce - (cs + w * K);
     {
         const int64_t ws = cs + w * K;
-        const float2 st = *global_at<float2>(ck_tile + (size_t)w * 64, ck_lane);
-        m = st.x;
-        c = st.y;
-        const uint64_t bits = het_mask(ri, rj, ws, a.W);
-        uint32_t d[K];
-        load_descs<K>(a.desc, ws, d);
-        if (ws == ps) d[0] = b.sd_fwd;
+        Window<K, NP> win;
+        load_window(a, L, ws, win);
+        if (ws == ps) win.d[0] = b.sd_fwd;
+#pragma unroll
+        for (int q = 0; q < NP; ++q) {
+            const float2 st = *ck_at(L, w, q);
+            m[q] = st.x;
+            c[q] = st.y;
+        }
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             if (k < cnt) {
-                fwd_segment(a.t, d[k], het_off_of(bits, k), m, c);
-                fm[k] = m;
-                fc[k] = c;
+#pragma unroll
+                for (int q = 0; q < NP; ++q) {
+                    fwd_segment(a.t, win.d[k], het_bit(win.bits[q], k), m[q], c[q]);
+                    fm[q][k] = m[q];
+                    fc[q][k] = c[q];
+                }
             }
         }
     }
@@ -381,34 +508,35 @@ __device__ __forceinline__ void decode_item(const FbArgs& a, int item_in, int la
     for (;;) {
         const bool has_next = w > 0;
         const int64_t ws = cs + w * K;
-        uint32_t dc[K], dn[K];
-        load_descs<K>(a.desc, ws, dc);
-        if (ws == ps) dc[0] = b.sd_bwd;
-        const uint64_t bc = het_mask(ri, rj, ws, a.W);
-        uint64_t bn = 0;
-        const int64_t wn = ws - K;
+        Window<K, NP> cur, nxt;
+        load_window(a, L, ws, cur);
+        if (ws == ps) cur.d[0] = b.sd_bwd;
         if (has_next) {
-            const float2 st = *global_at<float2>(ck_tile + (size_t)(w - 1) * 64, ck_lane);
-            m = st.x;
-            c = st.y;
-            load_descs<K>(a.desc, wn, dn);
-            if (wn == ps) dn[0] = b.sd_fwd;
-            bn = het_mask(ri, rj, wn, a.W);
+            load_window(a, L, ws - K, nxt);
+            if (ws - K == ps) nxt.d[0] = b.sd_fwd;
+#pragma unroll
+            for (int q = 0; q < NP; ++q) {
+                const float2 st = *ck_at(L, w - 1, q);
+                m[q] = st.x;
+                c[q] = st.y;
+            }
         } else {
 #pragma unroll
-            for (int k = 0; k < K; ++k) dn[k] = 0;
+            for (int k = 0; k < K; ++k) nxt.d[k] = 0;
+#pragma unroll
+            for (int q = 0; q < NP; ++q) nxt.bits[q] = 0;
         }
-        const bool fast = has_next && cnt == K && no_long_gaps<K>(dc) && no_long_gaps<K>(dn);
+        const bool fast = has_next && cnt == K && no_long_gaps<K>(cur.d) && no_long_gaps<K>(nxt.d);
         if (fast) {
             if (orient_b)
-                window_step<K, CI, POST, true, true>(a, j, ws, cnt, true, dc, bc, dn, bn, m, c, col, live, fm, fc, ym, yc);
+                window_step<K, NP, CI, POST, true, true>(a, j, L, ws, cnt, true, cur, nxt, m, c, fm, fc, ym, yc);
             else
-                window_step<K, CI, POST, false, true>(a, j, ws, cnt, true, dc, bc, dn, bn, m, c, col, live, fm, fc, ym, yc);
+                window_step<K, NP, CI, POST, false, true>(a, j, L, ws, cnt, true, cur, nxt, m, c, fm, fc, ym, yc);
         } else {
             if (orient_b)
-                window_step<K, CI, POST, true, false>(a, j, ws, cnt, has_next, dc, bc, dn, bn, m, c, col, live, fm, fc, ym, yc);
+                window_step<K, NP, CI, POST, true, false>(a, j, L, ws, cnt, has_next, cur, nxt, m, c, fm, fc, ym, yc);
             else
-                window_step<K, CI, POST, false, false>(a, j, ws, cnt, has_next, dc, bc, dn, bn, m, c, col, live, fm, fc, ym, yc);
+                window_step<K, NP, CI, POST, false, false>(a, j, L, ws, cnt, has_next, cur, nxt, m, c, fm, fc, ym, yc);
         }
         if (!has_next) break;
         orient_b = !orient_b;
@@ -417,32 +545,52 @@ __device__ __forceinline__ void decode_item(const FbArgs& a, int item_in, int la
     }
 }
 
-// Static schedule: wave w of workgroup g decodes item g * (WG / 64) + w.
-// OCC asks the compiler to fit at least OCC waves per SIMD (1: no floor).
-template <int K, bool CI, bool POST, int WG, int OCC>
-__global__ __launch_bounds__(WG, OCC) void fb_items_kernel(FbArgs a)
+// A decode is two launches over the same work items: fb_fwd_* (forward
+// walk + right-flank backward walk; few registers, so many waves per SIMD
+// to hide the latency of its single lookup chain) then fb_bwd_* (the
+// pipelined core windows). Static schedule: workgroup g decodes item g;
+// persistent schedule: waves take items from a device counter until it
+// passes n_items, so every wave exits.
+template <int K, int NP>
+__global__ __launch_bounds__(64) void fb_fwd_items(FbArgs a)
 {
-    const int lane = threadIdx.x & 63;
-    const int item = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (WG / 64) + (threadIdx.x >> 6)));
-    if (item < a.n_items) decode_item<K, CI, POST>(a, item, lane);
+    const int item = __builtin_amdgcn_readfirstlane((int)blockIdx.x);
+    if (item < a.n_items) forward_item<K, NP>(a, item, threadIdx.x);
 }
 
-// Persistent schedule: each wave takes items from a device counter until
-// the queue is drained; every wave exits once the counter passes n_items.
-template <int K, bool CI, bool POST, int WG, int OCC>
-__global__ __launch_bounds__(WG, OCC) void fb_queue_kernel(FbArgs a)
+template <int K, int NP, bool CI, bool POST>
+__global__ __launch_bounds__(64) void fb_bwd_items(FbArgs a)
 {
-    const int lane = threadIdx.x & 63;
-    for (;;) {
-        int item = 0;
-        if (lane == 0) item = atomicAdd(a.queue, 1);
-        item = __builtin_amdgcn_readfirstlane(__shfl(item, 0));
-        if (item >= a.n_items) break;
-        decode_item<K, CI, POST>(a, item, lane);
-    }
+    const int item = __builtin_amdgcn_readfirstlane((int)blockIdx.x);
+    if (item < a.n_items) backward_item<K, NP, CI, POST>(a, item, threadIdx.x);
 }
 
-__global__ void reset_counter_kernel(int* counter) { *counter = 0; }
+__device__ __forceinline__ int next_item(int* counter, int lane)
+{
+    int item = 0;
+    if (lane == 0) item = atomicAdd(counter, 1);
+    return __builtin_amdgcn_readfirstlane(__shfl(item, 0));
+}
+
+template <int K, int NP>
+__global__ __launch_bounds__(64) void fb_fwd_queue(FbArgs a)
+{
+    for (int item = next_item(a.queue, threadIdx.x); item < a.n_items; item = next_item(a.queue, threadIdx.x))
+        forward_item<K, NP>(a, item, threadIdx.x);
+}
+
+template <int K, int NP, bool CI, bool POST>
+__global__ __launch_bounds__(64) void fb_bwd_queue(FbArgs a)
+{
+    for (int item = next_item(a.queue2, threadIdx.x); item < a.n_items; item = next_item(a.queue2, threadIdx.x))
+        backward_item<K, NP, CI, POST>(a, item, threadIdx.x);
+}
+
+__global__ void reset_counters_kernel(int* counters)
+{
+    counters[0] = 0;
+    counters[1] = 0;
+}
 
 // Plain streaming copy (16 B per lane per step), used to calibrate HBM
 // traffic counters against a kernel of known size.

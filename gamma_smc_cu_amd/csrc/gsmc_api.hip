@@ -227,10 +227,9 @@ using namespace gsmc;
 
 struct Tuning {
     int K = 16;              // checkpoint interval = register window
-    int wg = 64;             // workgroup size (64 or 256 lanes)
+    int np = 1;              // pairs per lane (mean-only kernel; 1 or 2)
     int schedule = 0;        // 0 static, 1 persistent queue
     int xcd_order = 0;       // static schedule: give each XCD a contiguous run of items
-    int occ = 1;             // waves per SIMD the mean-only kernel is built to fit at least
     int64_t min_waves = 8192;
 };
 
@@ -266,7 +265,7 @@ struct gsmc_context {
 // unless it repeats the same metadata on the same stream.
 struct gsmc_batch {
     gsmc_context* ctx = nullptr;
-    DevBuf meta, ck, queue;
+    DevBuf meta, ck, queue, yb;
     PinnedBuf stage;
     std::vector<char> last;
     hipStream_t last_stream = nullptr;
@@ -389,7 +388,7 @@ struct Plan {
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-Plan make_plan(const gsmc_context* ctx, const std::vector<JobSpec>& jobs, const Tuning& tu, int wg)
+Plan make_plan(const gsmc_context* ctx, const std::vector<JobSpec>& jobs, const Tuning& tu, int np)
 {
     std::vector<JobDev> jd;
     std::vector<BlockDev> bd;
@@ -400,7 +399,8 @@ Plan make_plan(const gsmc_context* ctx, const std::vector<JobSpec>& jobs, const 
         if (js.P == 0 || js.blocks.empty()) continue;
         jd.push_back({js.out.mean, js.out.lower, js.out.upper, js.out.alpha, js.out.beta, js.pi, js.pj, js.P,
                       js.row0});
-        const int64_t tiles = (js.P + 63) / 64;
+        const int64_t tile_pairs = 64 * (int64_t)np;
+        const int64_t tiles = (js.P + tile_pairs - 1) / tile_pairs;
         for (const Block& b : js.blocks) {
             BlockDev d;
             d.cs = b.cs;
@@ -419,7 +419,7 @@ Plan make_plan(const gsmc_context* ctx, const std::vector<JobSpec>& jobs, const 
             d.job = (int32_t)(jd.size() - 1);
             d.nwin = (int32_t)((b.ce - b.cs + tu.K - 1) / tu.K);
             d.ck_off = (int64_t)ck;
-            ck += (size_t)tiles * d.nwin * 64;
+            ck += (size_t)tiles * d.nwin * (size_t)tile_pairs;
             require(bd.size() < (size_t)INT32_MAX, "too many block windows");
             for (int64_t t = 0; t < tiles; ++t) items.push_back({(int32_t)bd.size(), (int32_t)t});
             bd.push_back(d);
@@ -431,7 +431,7 @@ Plan make_plan(const gsmc_context* ctx, const std::vector<JobSpec>& jobs, const 
         // Workgroup g runs on XCD g % 8: hand each XCD a contiguous run of
         // workgroup-sized chunks, so the waves sharing an XCD's L2 walk the
         // same blocks (and so the same table layers) at the same time.
-        const int64_t wpw = wg / 64;
+        const int64_t wpw = 1;  // 64-lane workgroups: one item each
         const int64_t G = ((int64_t)items.size() + wpw - 1) / wpw;
         const int64_t per = (G + 7) / 8;
         std::vector<ItemDev> perm((size_t)(per * 8 * wpw), ItemDev{-1, 0});
@@ -456,60 +456,52 @@ Plan make_plan(const gsmc_context* ctx, const std::vector<JobSpec>& jobs, const 
     return pl;
 }
 
-template <int K, bool C, bool Q, int WG, int OCC>
-void launch_kernel(const FbArgs& a, int schedule, unsigned grid, hipStream_t st)
+// Pairs per lane a launch uses: the mean-only kernels (the throughput path)
+// are built for 1 and 2, the CI / posterior kernels for 1.
+int launch_np(const Tuning& tu, int flags) { return (flags & (GSMC_CI | GSMC_POSTERIOR)) ? 1 : tu.np; }
+
+// Kernel pair of one launch shape, and each kernel's resident waves.
+struct KernelPair {
+    const void* fwd_items;
+    const void* bwd_items;
+    const void* fwd_queue;
+    const void* bwd_queue;
+};
+
+template <int K, int NP, bool C, bool Q> KernelPair kernels_of()
 {
-    if (schedule == 1)
-        hipLaunchKernelGGL((fb_queue_kernel<K, C, Q, WG, OCC>), dim3(grid), dim3(WG), 0, st, a);
-    else
-        hipLaunchKernelGGL((fb_items_kernel<K, C, Q, WG, OCC>), dim3(grid), dim3(WG), 0, st, a);
+    return {reinterpret_cast<const void*>(&fb_fwd_items<K, NP>),
+            reinterpret_cast<const void*>(&fb_bwd_items<K, NP, C, Q>),
+            reinterpret_cast<const void*>(&fb_fwd_queue<K, NP>),
+            reinterpret_cast<const void*>(&fb_bwd_queue<K, NP, C, Q>)};
 }
 
-template <int K, bool C, bool Q, int WG, int OCC> const void* queue_fn()
-{
-    return reinterpret_cast<const void*>(&fb_queue_kernel<K, C, Q, WG, OCC>);
-}
-
-// The mean-only kernel (the throughput path) is built for every launch shape
-// the tuner sweeps; the CI / posterior kernels for the checkpoint interval
-// only, with 64-lane workgroups and no occupancy floor.
-template <int K> void dispatch_k(const FbArgs& a, const Tuning& tu, int flags, unsigned grid, hipStream_t st)
-{
-    const bool ci = flags & GSMC_CI, post = flags & GSMC_POSTERIOR;
-    if (ci && post) return launch_kernel<K, true, true, 64, 1>(a, tu.schedule, grid, st);
-    if (ci) return launch_kernel<K, true, false, 64, 1>(a, tu.schedule, grid, st);
-    if (post) return launch_kernel<K, false, true, 64, 1>(a, tu.schedule, grid, st);
-    constexpr int kOcc = K == 8 ? 6 : 1;  // the floor is built for K = 8 only
-    if (tu.wg == 256) {
-        if (tu.occ >= 6) return launch_kernel<K, false, false, 256, kOcc>(a, tu.schedule, grid, st);
-        return launch_kernel<K, false, false, 256, 1>(a, tu.schedule, grid, st);
-    }
-    if (tu.occ >= 6) return launch_kernel<K, false, false, 64, kOcc>(a, tu.schedule, grid, st);
-    return launch_kernel<K, false, false, 64, 1>(a, tu.schedule, grid, st);
-}
-
-template <int K> const void* queue_kernel_of(const Tuning& tu, int flags)
+template <int K> KernelPair kernels_k(const Tuning& tu, int flags)
 {
     const bool ci = flags & GSMC_CI, post = flags & GSMC_POSTERIOR;
-    if (ci && post) return queue_fn<K, true, true, 64, 1>();
-    if (ci) return queue_fn<K, true, false, 64, 1>();
-    if (post) return queue_fn<K, false, true, 64, 1>();
-    constexpr int kOcc = K == 8 ? 6 : 1;
-    if (tu.wg == 256) return tu.occ >= 6 ? queue_fn<K, false, false, 256, kOcc>() : queue_fn<K, false, false, 256, 1>();
-    return tu.occ >= 6 ? queue_fn<K, false, false, 64, kOcc>() : queue_fn<K, false, false, 64, 1>();
+    if (ci && post) return kernels_of<K, 1, true, true>();
+    if (ci) return kernels_of<K, 1, true, false>();
+    if (post) return kernels_of<K, 1, false, true>();
+    if (tu.np == 2) return kernels_of<K, 2, false, false>();
+    return kernels_of<K, 1, false, false>();
 }
 
-// Workgroup size a launch uses under tuning tu and flags.
-int launch_wg(const Tuning& tu, int flags) { return (flags & (GSMC_CI | GSMC_POSTERIOR)) ? 64 : tu.wg; }
+KernelPair kernels_for(const Tuning& tu, int flags)
+{
+    return tu.K == 8 ? kernels_k<8>(tu, flags) : kernels_k<16>(tu, flags);
+}
 
-// Waves of the decode kernel one device holds at once (persistent grid size).
-int64_t resident_waves(const gsmc_context* ctx, const Tuning& tu, int flags)
+int64_t resident(const gsmc_context* ctx, const void* fn)
 {
     int per_cu = 0;
-    const void* fn = tu.K == 8 ? queue_kernel_of<8>(tu, flags) : queue_kernel_of<16>(tu, flags);
-    const int wg = launch_wg(tu, flags);
-    HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, wg, 0));
-    return (int64_t)std::max(per_cu, 1) * ctx->cus * (wg / 64);
+    HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64, 0));
+    return (int64_t)std::max(per_cu, 1) * ctx->cus;
+}
+
+void launch(const void* fn, unsigned grid, const FbArgs& a, hipStream_t st)
+{
+    void* args[] = {const_cast<FbArgs*>(&a)};
+    HIP_OK(hipLaunchKernel(fn, dim3(grid), dim3(64), args, 0, st));
 }
 
 // Queue one launch of `jobs` on `stream` using workspace b.
@@ -517,7 +509,7 @@ void launch_plan(gsmc_batch* b, const std::vector<JobSpec>& jobs, int flags, hip
 {
     gsmc_context* ctx = b->ctx;
     const Tuning tu = ctx->tune;
-    Plan pl = make_plan(ctx, jobs, tu, launch_wg(tu, flags));
+    Plan pl = make_plan(ctx, jobs, tu, launch_np(tu, flags));
     if (pl.n_items == 0) return;
     const bool same = b->pending && stream == b->last_stream && pl.bytes == b->last;
     if (!same) {
@@ -526,6 +518,7 @@ void launch_plan(gsmc_batch* b, const std::vector<JobSpec>& jobs, int flags, hip
         b->meta.ensure(pl.bytes.size(), ctx->device);
         b->ck.ensure(std::max<size_t>(pl.ck_elems, 1) * sizeof(float2), ctx->device);
         b->queue.ensure(256, ctx->device);
+        b->yb.ensure((size_t)std::max<int64_t>(pl.n_items, 1) * 64 * 2 * sizeof(float2), ctx->device);
         b->stage.ensure(pl.bytes.size());
         std::memcpy(b->stage.p, pl.bytes.data(), pl.bytes.size());
         HIP_OK(hipMemcpyAsync(b->meta.p, b->stage.p, pl.bytes.size(), hipMemcpyHostToDevice, stream));
@@ -542,19 +535,20 @@ void launch_plan(gsmc_batch* b, const std::vector<JobSpec>& jobs, int flags, hip
     a.items = reinterpret_cast<const ItemDev*>(b->meta.as<char>() + pl.off_items);
     a.n_items = pl.n_items;
     a.queue = b->queue.as<int>();
+    a.queue2 = b->queue.as<int>() + 1;
     a.ck = b->ck.as<float2>();
+    a.yb = b->yb.as<float2>();
     a.t = ctx->tabs->view;
     a.unscale = 2.0f * ctx->Ne;
-    const int64_t wpw = launch_wg(tu, flags) / 64;
-    unsigned grid;
+    const KernelPair kp = kernels_for(tu, flags);
     if (tu.schedule == 1) {
-        hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, stream, a.queue);
-        grid = (unsigned)std::max<int64_t>(1, std::min(resident_waves(ctx, tu, flags), pl.n_items) / wpw);
+        hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(1), 0, stream, a.queue);
+        launch(kp.fwd_queue, (unsigned)std::max<int64_t>(1, std::min(resident(ctx, kp.fwd_queue), pl.n_items)), a, stream);
+        launch(kp.bwd_queue, (unsigned)std::max<int64_t>(1, std::min(resident(ctx, kp.bwd_queue), pl.n_items)), a, stream);
     } else {
-        grid = (unsigned)((pl.n_items + wpw - 1) / wpw);
+        launch(kp.fwd_items, (unsigned)pl.n_items, a, stream);
+        launch(kp.bwd_items, (unsigned)pl.n_items, a, stream);
     }
-    if (tu.K == 8) dispatch_k<8>(a, tu, flags, grid, stream);
-    else dispatch_k<16>(a, tu, flags, grid, stream);
     HIP_OK(hipGetLastError());
     HIP_OK(hipEventRecord(b->done, stream));
     b->pending = true;
@@ -608,7 +602,8 @@ void run_to_host(gsmc_context* ctx, const int32_t* pi, const int32_t* pj, int64_
     // by a device budget for the two output slots.
     size_t free_b = 0, total_b = 0;
     HIP_OK(hipMemGetInfo(&free_b, &total_b));
-    const int64_t tiles = (chunk + 63) / 64;
+    const int64_t tile_pairs = 64 * (int64_t)launch_np(ctx->tune, flags);
+    const int64_t tiles = (chunk + tile_pairs - 1) / tile_pairs;
     int64_t per_group = std::max<int64_t>(1, (ctx->tune.min_waves + tiles - 1) / tiles);
     int64_t max_rows = 0;
     for (const Block& b : blocks) max_rows = std::max<int64_t>(max_rows, b.ce - b.cs);
@@ -883,22 +878,20 @@ int gsmc_context_info(const gsmc_context* ctx, int64_t* n, int64_t* S, int* devi
     });
 }
 
-int gsmc_context_set_tuning(gsmc_context* ctx, int k, int workgroup, int schedule, int xcd_order, int occupancy,
+int gsmc_context_set_tuning(gsmc_context* ctx, int k, int pairs_per_lane, int schedule, int xcd_order,
                             int64_t min_waves)
 {
     return guarded([&] {
-        require(occupancy == 1 || (occupancy == 6 && k == 8), "occupancy must be 1 (no floor), or 6 with checkpoint_k 8");
         require(ctx != nullptr, "context is NULL");
         require(k == 8 || k == 16, "checkpoint_k must be 8 or 16");
-        require(workgroup == 64 || workgroup == 256, "workgroup must be 64 or 256");
+        require(pairs_per_lane == 1 || pairs_per_lane == 2, "pairs_per_lane must be 1 or 2");
         require(schedule == 0 || schedule == 1, "schedule must be 0 (static) or 1 (persistent)");
         require(min_waves >= 1, "min_waves_per_launch must be positive");
         std::lock_guard<std::mutex> lk(ctx->mu);
         ctx->tune.K = k;
-        ctx->tune.wg = workgroup;
+        ctx->tune.np = pairs_per_lane;
         ctx->tune.schedule = schedule;
         ctx->tune.xcd_order = xcd_order ? 1 : 0;
-        ctx->tune.occ = occupancy;
         ctx->tune.min_waves = min_waves;
     });
 }
@@ -1083,11 +1076,12 @@ int gsmc_checkpoint_bytes(gsmc_context* ctx, int64_t n_sites, int64_t core, int6
 {
     return guarded([&] {
         require(ctx != nullptr && bytes != nullptr, "NULL argument");
-        const int64_t tiles = (n_pairs + 63) / 64;
+        const int64_t tile_pairs = 64 * (int64_t)ctx->tune.np;
+        const int64_t tiles = (n_pairs + tile_pairs - 1) / tile_pairs;
         uint64_t windows = 0;
         for (const Block& b : make_blocks(n_sites, core > 0 ? core : std::max<int64_t>(n_sites, 1), core > 0 ? flank : 0))
             windows += (uint64_t)((b.ce - b.cs + ctx->tune.K - 1) / ctx->tune.K);
-        *bytes = windows * (uint64_t)tiles * 64 * sizeof(float2);
+        *bytes = windows * (uint64_t)tiles * (uint64_t)tile_pairs * sizeof(float2);
     });
 }
 

@@ -115,17 +115,16 @@ void gsmc_context_destroy(gsmc_context* ctx);
 int gsmc_context_info(const gsmc_context* ctx, int64_t* n_haps, int64_t* n_sites, int* device_id, float* Ne);
 
 /* Decoder tuning; results do not depend on any of it. checkpoint_k: sites
- * per checkpoint window, 8 or 16 (default 16); workgroup: 64 or 256 lanes
- * (default 64); schedule: 0 static grid of work items, 1 persistent waves
- * draining a device queue (default 0); xcd_order: static grid ordered so
- * each XCD walks a contiguous run of items (default 0); occupancy: 1 (no
- * floor) or 6 waves per SIMD the mean-only kernel is compiled to fit
- * (default 1); min_waves: work items the host-output path puts in one
- * launch at least (default 8192). Workgroup and occupancy apply to the
- * mean-only kernel; CI / posterior launches use 64 lanes and no floor.
- * No reference counterpart (the reference has fixed 256-thread launches). */
-int gsmc_context_set_tuning(gsmc_context* ctx, int checkpoint_k, int workgroup, int schedule, int xcd_order,
-                            int occupancy, int64_t min_waves_per_launch);
+ * per checkpoint window, 8 or 16 (default 16); pairs_per_lane: 1 or 2
+ * (default 1), i.e. 64 or 128 pairs per 64-lane workgroup, for the
+ * mean-only kernel (CI / posterior launches use 1); schedule: 0 static grid
+ * of work items, 1 persistent waves draining a device queue (default 0);
+ * xcd_order: static grid ordered so each XCD walks a contiguous run of
+ * items (default 0); min_waves: work items the host-output path puts in
+ * one launch at least (default 8192). No reference counterpart (the
+ * reference has fixed 256-thread launches). */
+int gsmc_context_set_tuning(gsmc_context* ctx, int checkpoint_k, int pairs_per_lane, int schedule, int xcd_order,
+                            int64_t min_waves_per_launch);
 
 /* Full-sequence forward-backward of n_pairs pairs (pair_i[p], pair_j[p]) into
  * host arrays laid out site-major [n_sites][n_pairs]: mean always; lower and

@@ -144,10 +144,9 @@ def test_pair_index_out_of_range_raises(core, cohort_small, flow_field_path):
         ctx.run_fb([(0, G.shape[0])])
 
 
-SCHEDULES = [dict(checkpoint_k=k, workgroup=wg, schedule=sch, xcd_order=xcd, occupancy=occ)
-             for k in (8, 16) for occ in (1, 6) for wg in (64, 256)
-             for sch, xcd in (("static", True), ("static", False), ("persistent", False))
-             if occ == 1 or k == 8]
+SCHEDULES = [dict(checkpoint_k=k, pairs_per_lane=np_, schedule=sch, xcd_order=xcd)
+             for k in (8, 16) for np_ in (1, 2)
+             for sch, xcd in (("static", True), ("static", False), ("persistent", False))]
 
 
 def test_launch_shapes_are_bit_identical(core, cohort_small, flow_field_path):
@@ -182,7 +181,7 @@ def _ragged_jobs(n_haps, S, rng):
     return jobs
 
 
-@pytest.mark.parametrize("tu", [SCHEDULES[0], SCHEDULES[5], SCHEDULES[8], SCHEDULES[-1]])
+@pytest.mark.parametrize("tu", [SCHEDULES[0], SCHEDULES[4], SCHEDULES[8], SCHEDULES[-1]])
 def test_batch_matches_alone_and_oracle(core, cohort_small, flow_field_path, tu):
     from gamma_smc_cu_amd import batch
 
