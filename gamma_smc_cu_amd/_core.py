@@ -38,7 +38,9 @@ EXPORTS = (
     "gsmc_memcpy_dtoh", "gsmc_memset", "gsmc_synchronize", "gsmc_stream_create", "gsmc_stream_destroy",
     "gsmc_stream_synchronize", "gsmc_event_create", "gsmc_event_destroy", "gsmc_event_record",
     "gsmc_event_elapsed_ms", "gsmc_batch_create", "gsmc_batch_destroy", "gsmc_batch_launch", "gsmc_batch_wait",
-    "gsmc_batch_query", "gsmc_checkpoint_bytes", "gsmc_device_copy",
+    "gsmc_batch_query", "gsmc_checkpoint_bytes", "gsmc_device_copy", "gsmc_memcpy_htod_async",
+    "gsmc_memcpy_dtoh_async", "gsmc_host_alloc", "gsmc_host_free", "gsmc_event_query",
+    "gsmc_batch_reserve",
 )
 
 
@@ -99,6 +101,12 @@ def _declare(L):
         "gsmc_batch_query": [_vp, ctypes.POINTER(ctypes.c_int)],
         "gsmc_checkpoint_bytes": [_vp, _i64, _i64, _i64, _i64, ctypes.POINTER(ctypes.c_uint64)],
         "gsmc_device_copy": [_vp, _vp, ctypes.c_uint64, _vp],
+        "gsmc_memcpy_htod_async": [_vp, _vp, ctypes.c_uint64, _vp],
+        "gsmc_memcpy_dtoh_async": [_vp, _vp, ctypes.c_uint64, _vp],
+        "gsmc_host_alloc": [ctypes.POINTER(_vp), ctypes.c_uint64],
+        "gsmc_host_free": [_vp],
+        "gsmc_event_query": [_vp, ctypes.POINTER(ctypes.c_int)],
+        "gsmc_batch_reserve": [_vp, ctypes.c_uint64, _i64, _i64],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -202,6 +210,11 @@ def build_tables(flow_field_path, n_steps, Ne, mu, rho, table_precision="fp32"):
 
 
 def _split_pairs(pairs):
+    if isinstance(pairs, np.ndarray) and pairs.ndim == 2 and pairs.shape[1] == 2 and pairs.dtype.kind in "iu":
+        if pairs.size and (pairs.min() < np.iinfo(np.int32).min or pairs.max() > np.iinfo(np.int32).max):
+            raise ValueError("pair index out of range")
+        return (np.ascontiguousarray(pairs[:, 0], dtype=np.int32),
+                np.ascontiguousarray(pairs[:, 1], dtype=np.int32))
     pairs = list(pairs)
     pi = np.ascontiguousarray([int(p[0]) for p in pairs], dtype=np.int32)
     pj = np.ascontiguousarray([int(p[1]) for p in pairs], dtype=np.int32)

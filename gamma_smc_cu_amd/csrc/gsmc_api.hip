@@ -1003,6 +1003,26 @@ int gsmc_batch_create(gsmc_context* ctx, gsmc_batch** out)
 
 void gsmc_batch_destroy(gsmc_batch* b) { delete b; }
 
+int gsmc_batch_reserve(gsmc_batch* b, uint64_t checkpoint_bytes, int64_t max_items, int64_t max_jobs)
+{
+    return guarded([&] {
+        require(b != nullptr, "batch is NULL");
+        require(max_items >= 0 && max_jobs >= 0, "negative reservation");
+        gsmc_context* ctx = b->ctx;
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        HIP_OK(hipSetDevice(ctx->device));
+        if (b->pending) HIP_OK(hipEventSynchronize(b->done));
+        // A launch's metadata: jobs, then at most one block per item, then items.
+        const size_t meta = align256((size_t)max_jobs * sizeof(JobDev)) +
+                            align256((size_t)max_items * sizeof(BlockDev)) + (size_t)max_items * sizeof(ItemDev);
+        b->meta.ensure(meta, ctx->device);
+        b->stage.ensure(meta);
+        b->ck.ensure(std::max<uint64_t>(checkpoint_bytes, 1), ctx->device);
+        b->queue.ensure(256, ctx->device);
+        b->yb.ensure((size_t)std::max<int64_t>(max_items, 1) * 64 * 2 * sizeof(float2), ctx->device);
+    });
+}
+
 int gsmc_batch_launch(gsmc_batch* b, const gsmc_job* jobs, int64_t n_jobs, int flags, void* stream)
 {
     return guarded([&] {
@@ -1114,6 +1134,26 @@ int gsmc_memcpy_dtoh(void* dst, const void* src, uint64_t bytes)
 {
     return guarded([&] { HIP_OK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost)); });
 }
+int gsmc_memcpy_htod_async(void* dst, const void* src, uint64_t bytes, void* stream)
+{
+    return guarded([&] {
+        HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, static_cast<hipStream_t>(stream)));
+    });
+}
+int gsmc_memcpy_dtoh_async(void* dst, const void* src, uint64_t bytes, void* stream)
+{
+    return guarded([&] {
+        HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, static_cast<hipStream_t>(stream)));
+    });
+}
+int gsmc_host_alloc(void** ptr, uint64_t bytes)
+{
+    return guarded([&] { HIP_OK(hipHostMalloc(ptr, std::max<uint64_t>(bytes, 1), hipHostMallocDefault)); });
+}
+int gsmc_host_free(void* ptr)
+{
+    return guarded([&] { HIP_OK(hipHostFree(ptr)); });
+}
 int gsmc_memset(void* dst, int value, uint64_t bytes)
 {
     return guarded([&] { HIP_OK(hipMemset(dst, value, bytes)); });
@@ -1153,6 +1193,18 @@ int gsmc_event_destroy(void* event)
 int gsmc_event_record(void* event, void* stream)
 {
     return guarded([&] { HIP_OK(hipEventRecord(static_cast<hipEvent_t>(event), static_cast<hipStream_t>(stream))); });
+}
+int gsmc_event_query(void* event, int* done)
+{
+    return guarded([&] {
+        const hipError_t e = hipEventQuery(static_cast<hipEvent_t>(event));
+        if (e == hipErrorNotReady) {
+            *done = 0;
+            return;
+        }
+        HIP_OK(e);
+        *done = 1;
+    });
 }
 int gsmc_event_elapsed_ms(void* start, void* stop, float* ms)
 {

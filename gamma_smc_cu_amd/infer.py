@@ -72,6 +72,25 @@ def _estimate_scaled_params(G, positions, mu, rho, Ne):
     return pi_hat / (4.0 * float(Ne)), pi_hat * ratio / (4.0 * float(Ne))
 
 
+def _estimate_scaled_params_packed(packed, n_haps, positions, mu, rho, Ne):
+    """:func:`_estimate_scaled_params` on bitpacked rows (``packed[h][w]``
+    bit ``s & 63`` of word ``s >> 6``, zero tail bits): the per-diploid
+    heterozygous site counts are popcounts of ``row(2k) ^ row(2k+1)``."""
+    S = int(positions.shape[0])
+    if n_haps < 2 or S == 0 or n_haps % 2 != 0:
+        return float(mu), float(rho)
+    span = float(positions[-1] + 1.0)
+    if span <= 0:
+        return float(mu), float(rho)
+    rows = np.asarray(packed, dtype=np.uint64)
+    het = np.bitwise_count(rows[0:n_haps:2] ^ rows[1:n_haps:2]).sum(axis=1, dtype=np.int64)
+    pi_hat = float(het.mean()) / span
+    if not (pi_hat > 0 and math.isfinite(pi_hat)):
+        return float(mu), float(rho)
+    ratio = float(rho) / max(float(mu), 1e-30)
+    return pi_hat / (4.0 * float(Ne)), pi_hat * ratio / (4.0 * float(Ne))
+
+
 def _filter_segregating(G, positions):
     """Drop sites whose allele count is 0 or n (``infer.py:102-126``); no copy
     when every site segregates."""

@@ -81,3 +81,68 @@ CHROMOSOME = dict(n=356, S=402_853, span=51_000_000, seed=7)      # 63,190 pairs
 BIOBANK = dict(n=1000, S=50_000, span=7_500_000, seed=17)         # 499,500 pairs
 SERVING = dict(n=6404, S=1_000_000, span=150_000_000, seed=13)    # resident cohort for the request loop
 RAGGED_COHORT = dict(n=2000, seed=11, bp_per_site=150)             # batched ragged jobs
+
+
+def ragged_jobs(n_haps, n_sites, n_jobs=64, seed=11, sites=(2_000, 200_000), pairs=(100, 20_000)):
+    """Ragged decode jobs over a cohort of ``n_haps`` x ``n_sites``.
+
+    Job k draws, in order, from ``numpy.random.default_rng([seed, 1])``: its
+    site count ``exp(U(log lo, log hi))`` (rounded, at most ``n_sites``), its
+    pair count likewise, the first site uniformly among the starts that fit,
+    and its pairs as distinct indices into the all-pairs order (pair
+    ``p = i(i-1)/2 + j``, ``i > j``). Returns a list of
+    ``(site_lo, site_hi, pairs int32 (P, 2))``."""
+    rng = np.random.default_rng([seed, 1])
+    total = n_haps * (n_haps - 1) // 2
+    out = []
+    for _ in range(n_jobs):
+        S = int(min(n_sites, round(np.exp(rng.uniform(np.log(sites[0]), np.log(sites[1]))))))
+        P = int(min(total, round(np.exp(rng.uniform(np.log(pairs[0]), np.log(pairs[1]))))))
+        lo = int(rng.integers(0, n_sites - S + 1))
+        idx = np.sort(rng.choice(total, size=P, replace=False))
+        i = np.floor((1.0 + np.sqrt(1.0 + 8.0 * idx.astype(np.float64))) / 2.0).astype(np.int64)
+        i -= (i * (i - 1) // 2 > idx)
+        i += ((i + 1) * i // 2 <= idx)
+        j = idx - i * (i - 1) // 2
+        out.append((lo, lo + S, np.stack([i, j], 1).astype(np.int32)))
+    return out
+
+
+def resident_cohort(n, S, span, seed, chunk=2048, threads=None):
+    """Bitpacked 1/k-spectrum cohort for large ``n`` (the serving cohort):
+    ``(packed uint64 (n, W), positions float64 (S,))``.
+
+    Positions and counts are drawn as in :func:`cohort` from
+    ``default_rng(seed)``. Carriers of the sites in chunk ``c`` (``chunk``
+    sites each) come from their own stream ``default_rng([seed, 2, c])``:
+    one ``random((chunk, n), float32)`` key matrix, and a site's carriers
+    are its ``k`` smallest keys. The chunks are independent, so they are
+    drawn on a thread pool and the result does not depend on ``threads``.
+    """
+    from concurrent.futures import ThreadPoolExecutor
+
+    if chunk % 64:
+        raise ValueError("chunk must be a multiple of 64")
+    rng = np.random.default_rng(seed)
+    pos = _positions(rng, S, span)
+    counts = _counts(rng, n, S)
+    W = (S + 63) // 64
+    packed = np.zeros((n, W), np.uint64)
+    cols = np.arange(n)[None, :]
+
+    def one(c):
+        s0, s1 = c * chunk, min(S, (c + 1) * chunk)
+        keys = np.random.default_rng([seed, 2, c]).random((s1 - s0, n), dtype=np.float32)
+        order = keys.argsort(axis=1)
+        carr = np.zeros(keys.shape, bool)
+        np.put_along_axis(carr, order, cols < counts[s0:s1, None], axis=1)
+        m = s1 - s0
+        mw = (m + 63) // 64
+        bits = np.zeros((n, mw * 64), bool)
+        bits[:, :m] = carr.T
+        packed[:, s0 // 64:s0 // 64 + mw] = np.packbits(bits, axis=1, bitorder="little").view("<u8")
+
+    n_chunks = (S + chunk - 1) // chunk
+    with ThreadPoolExecutor(max_workers=threads or min(16, len(__import__("os").sched_getaffinity(0)))) as ex:
+        list(ex.map(one, range(n_chunks)))
+    return packed, pos

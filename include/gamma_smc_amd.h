@@ -176,6 +176,10 @@ void gsmc_batch_destroy(gsmc_batch* batch);
 int gsmc_batch_launch(gsmc_batch* batch, const gsmc_job* jobs, int64_t n_jobs, int flags, void* stream);
 int gsmc_batch_wait(gsmc_batch* batch);
 int gsmc_batch_query(gsmc_batch* batch, int* done);
+/* Grow a batch's buffers now (checkpoint bytes, work items and jobs of the
+ * largest launch the caller will make), so launches within these bounds
+ * never allocate or synchronise the device. */
+int gsmc_batch_reserve(gsmc_batch* batch, uint64_t checkpoint_bytes, int64_t max_items, int64_t max_jobs);
 
 /* Device bytes of checkpoints one job of n_pairs pairs over n_sites sites
  * needs under the context's tuning (for admission control). */
@@ -188,6 +192,13 @@ int gsmc_malloc(void** ptr, uint64_t bytes);
 int gsmc_free(void* ptr);
 int gsmc_memcpy_htod(void* dst, const void* src, uint64_t bytes);
 int gsmc_memcpy_dtoh(void* dst, const void* src, uint64_t bytes);
+/* Copies queued on `stream`; the host side must be pinned (gsmc_host_alloc)
+ * for the copy to overlap device work. */
+int gsmc_memcpy_htod_async(void* dst, const void* src, uint64_t bytes, void* stream);
+int gsmc_memcpy_dtoh_async(void* dst, const void* src, uint64_t bytes, void* stream);
+/* Pinned host memory. */
+int gsmc_host_alloc(void** ptr, uint64_t bytes);
+int gsmc_host_free(void* ptr);
 int gsmc_memset(void* dst, int value, uint64_t bytes);
 int gsmc_synchronize(void);
 int gsmc_stream_create(void** stream);
@@ -196,6 +207,7 @@ int gsmc_stream_synchronize(void* stream);
 int gsmc_event_create(void** event);
 int gsmc_event_destroy(void* event);
 int gsmc_event_record(void* event, void* stream);
+int gsmc_event_query(void* event, int* done); /* done = 1 once the event completed, without blocking */
 int gsmc_event_elapsed_ms(void* start, void* stop, float* ms);
 /* Streaming device-to-device copy kernel (bytes a multiple of 16), queued
  * on `stream`; a kernel of known traffic for counter calibration. */

@@ -338,6 +338,24 @@ def fb_blockwise_f32(G, positions, pairs, tables, Ne, core, flank, ci=False, pos
     return out
 
 
+def fb_blockwise_f64(G, positions, pairs, tables64, Ne, core, flank, exact_pow10=False):
+    """FP64 blockwise decode (same windows as :func:`fb_blockwise_f32`);
+    returns the float64 ``mean``."""
+    S = G.shape[1]
+    out = np.zeros((S, len(pairs)))
+    for cs, ce, ps, pe in block_windows(S, core, flank):
+        r = fb_window_f64(G, positions, pairs, tables64, Ne, ps, pe - ps, exact_pow10)
+        out[cs:ce] = r["mean"][cs - ps:ce - ps]
+    return {"mean": out}
+
+
+def log_error_report(mean32, mean64):
+    """|log mean32 - log mean64| summary (max, p99, median, mean)."""
+    e = np.abs(np.log(np.asarray(mean32, np.float64)) - np.log(np.asarray(mean64, np.float64)))
+    return {"max": float(e.max()), "p99": float(np.percentile(e, 99)), "median": float(np.median(e)),
+            "mean": float(e.mean()), "n": int(e.size)}
+
+
 def bitpack(G):
     """Bit b of word w of row h is ``G[h, 64 w + b] & 1``; tail bits zero
     (``bitpack.cu:11-30``)."""
