@@ -43,11 +43,11 @@ WORKLOADS = {
     "ragged": dict(n=2000, S=262_144, span=262_144 * 150, seed=11, flank=2048, core=32_768, n_jobs=64,
                    label="64 ragged jobs (sites logU[2k,200k] x pairs logU[100,20k]) over a 2,000-haplotype "
                          "1/k cohort at 1 SNP/150 bp, seed 11; 32,768-site cores + 2048-site flanks, mean_only"),
-    "serving": dict(n=6404, S=1_000_000, span=150_000_000, seed=13, arena_gb=96.0, ck_gb=8.0,
+    "serving": dict(n=6404, S=1_000_000, span=150_000_000, seed=13, arena_gb=96.0, ck_gb=8.0, core=8192,
                     rates=(50.0, 200.0, 800.0, 3200.0), seconds=4.0, max_queries=600,
                     label="resident 6,404 x 1,000,000 bitpacked 1/k cohort (seed 13, 150 bp/site); Poisson "
                           "queries: 4-200 haplotypes (all pairs) x 50 kb-5 Mb regions (log-uniform), "
-                          "32,768-site cores + 2048-site flanks, mean_only"),
+                          "8,192-site cores + 2048-site flanks (run_fb_blockwise defaults), mean_only"),
 }
 
 
@@ -304,7 +304,7 @@ def run_serving(args, wl):
     arena = int(wl["arena_gb"] * 1e9)
 
     def one_run(mode, rate, nq):
-        qs, off = serving.poisson_queries(nq, rate, n, S, wl["seed"])
+        qs, off = serving.poisson_queries(nq, rate, n, S, wl["seed"], core_block_sites=wl["core"])
         if mode == "batched":
             srv = serving.Server(ctx, arena, n_slots=2, ck_bytes_per_launch=int(wl["ck_gb"] * 1e9))
         else:
@@ -334,7 +334,7 @@ def run_serving(args, wl):
         rows.append(one_run("single", rate, nq))
 
     # equality check: the same queries alone vs in the mix (bounded sample)
-    qs, off = serving.poisson_queries(12, 1e6, n, S, wl["seed"] + 1)
+    qs, off = serving.poisson_queries(12, 1e6, n, S, wl["seed"] + 1, core_block_sites=wl["core"])
     qs = [q for q in qs if len(q.haplotypes) <= 60][:6] or qs[:2]
     mix = serving.Server(ctx, arena, n_slots=2, ck_bytes_per_launch=int(wl["ck_gb"] * 1e9))
     res = {}
@@ -411,6 +411,8 @@ def main():
                     help="take the launch shape from the config.tuning of a previous bench JSON line")
     ap.add_argument("--fp64-pairs", type=int, default=None,
                     help="pairs of the sample whose FP32 posterior is compared with the FP64 oracle")
+    ap.add_argument("--summary-steps", type=int, default=2,
+                    help="also time the per-site summary mode (run_fb_summary) this many times")
     ap.add_argument("--no-host-stream", action="store_true",
                     help="skip the host-output (PCIe) run of the biobank workload")
     ap.add_argument("--copy-calibration-gib", type=float, default=1.0,
@@ -513,6 +515,19 @@ def main():
         del src, dst
     achieved = B / (ms / 1e3) / 1e9
 
+    summary = None
+    if args.summary_steps > 0:
+        # Fused per-site reduction over all pairs (no per-pair output): the
+        # same decode with a 16 B per 64-pair group and site epilogue.
+        ctx.run_fb_summary(pairs, core_block_sites=core, flank_sites=flank)
+        t1 = time.perf_counter()
+        for _ in range(args.summary_steps):
+            ctx.run_fb_summary(pairs, core_block_sites=core, flank_sites=flank)
+        sm = (time.perf_counter() - t1) * 1e3 / args.summary_steps
+        summary = {"ms": sm, "pair_sites_per_s": pair_sites / (sm / 1e3), "steps": args.summary_steps,
+                   "timing": "host wall clock per call (pair upload, decode, 3 x n_sites floats back)"}
+        log(f"summary mode: {sm:.2f} ms/call")
+
     checks = {}
     cpu = fp64 = host = None
     if args.fp64_pairs is None:
@@ -549,6 +564,7 @@ def main():
         "cpu_baseline": cpu,
         "fp64_error": fp64,
         "host_stream": host,
+        "site_summary": summary,
         "checks": checks,
         "wall_ms_per_step": wall * 1e3 / args.steps,
         "copy_calibration_bytes": copy_bytes,

@@ -27,6 +27,7 @@ LIB_PATH = os.path.join(_HERE, "libgsmc.so")
 
 GSMC_CI = 1
 GSMC_POSTERIOR = 2
+GSMC_SUMMARY = 4
 _E_INVAL, _E_HIP, _E_NOMEM, _E_IO = -1, -2, -3, -4
 
 # Every symbol include/gamma_smc_amd.h declares.
@@ -40,7 +41,7 @@ EXPORTS = (
     "gsmc_event_elapsed_ms", "gsmc_batch_create", "gsmc_batch_destroy", "gsmc_batch_launch", "gsmc_batch_wait",
     "gsmc_batch_query", "gsmc_checkpoint_bytes", "gsmc_device_copy", "gsmc_memcpy_htod_async",
     "gsmc_memcpy_dtoh_async", "gsmc_host_alloc", "gsmc_host_free", "gsmc_event_query",
-    "gsmc_batch_reserve",
+    "gsmc_batch_reserve", "gsmc_run_fb_summary", "gsmc_run_fwd", "gsmc_run_fwd_device",
 )
 
 
@@ -50,7 +51,8 @@ class Job(ctypes.Structure):
     _fields_ = [("site_lo", ctypes.c_int64), ("site_hi", ctypes.c_int64), ("core_block_sites", ctypes.c_int64),
                 ("flank_sites", ctypes.c_int64), ("pair_i", ctypes.c_void_p), ("pair_j", ctypes.c_void_p),
                 ("n_pairs", ctypes.c_int64), ("mean", ctypes.c_void_p), ("lower", ctypes.c_void_p),
-                ("upper", ctypes.c_void_p), ("alpha", ctypes.c_void_p), ("beta", ctypes.c_void_p)]
+                ("upper", ctypes.c_void_p), ("alpha", ctypes.c_void_p), ("beta", ctypes.c_void_p),
+                ("site_mean", ctypes.c_void_p), ("site_min", ctypes.c_void_p), ("site_max", ctypes.c_void_p)]
 
 _lib = None
 _lock = threading.Lock()
@@ -107,6 +109,9 @@ def _declare(L):
         "gsmc_host_free": [_vp],
         "gsmc_event_query": [_vp, ctypes.POINTER(ctypes.c_int)],
         "gsmc_batch_reserve": [_vp, ctypes.c_uint64, _i64, _i64],
+        "gsmc_run_fb_summary": [_vp, _vp, _vp, _i64, _i64, _i64, _vp, _vp, _vp],
+        "gsmc_run_fwd": [_vp, _vp, _vp, _i64, _i32, _vp, _vp, _vp],
+        "gsmc_run_fwd_device": [_vp, _vp, _vp, _i64, _i32, _vp, _vp, _vp, _vp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -300,6 +305,40 @@ class FlowContext:
         sched = {"static": 0, "persistent": 1}[schedule]
         check(lib().gsmc_context_set_tuning(self._h, int(checkpoint_k), int(pairs_per_lane), sched,
                                             int(bool(xcd_order)), int(min_waves_per_launch)))
+
+    def run_fwd(self, pairs, mean_only=True):
+        """Forward-only filtering over the full sequence
+        (``bindings.cpp:2794-2849``): ``mean`` and, unless ``mean_only``,
+        ``lower`` / ``upper``, site-major ``(n_sites, n_pairs)`` float32."""
+        pi, pj = _split_pairs(pairs)
+        P = pi.shape[0]
+        out = {"mean": np.zeros((self._S, P), np.float32)}
+        if not mean_only:
+            out["lower"] = np.zeros((self._S, P), np.float32)
+            out["upper"] = np.zeros((self._S, P), np.float32)
+        if P and self._S:
+            check(lib().gsmc_run_fwd(self._h, _ptr(pi), _ptr(pj), P, 0 if mean_only else GSMC_CI, _ptr(out["mean"]),
+                                     _ptr(out.get("lower")), _ptr(out.get("upper"))))
+        return out
+
+    def run_fb_summary(self, pairs, core_block_sites=0, flank_sites=0):
+        """Per-site mean, min and max of the posterior mean over ``pairs``,
+        without per-pair outputs (``bindings.cpp:2695-2788``): returns
+        ``site_mean``, ``site_min``, ``site_max`` (float32, one per site) and
+        ``n_pairs``; no pairs gives an empty ``site_mean`` (and ``n_pairs`` 0). The reference decodes the full sequence;
+        ``core_block_sites > 0`` decodes block windows instead."""
+        pi, pj = _split_pairs(pairs)
+        P = pi.shape[0]
+        if P == 0:
+            return {"site_mean": np.zeros(0, np.float32), "n_pairs": 0}
+        if flank_sites < 0:
+            raise ValueError("flank_sites must be non-negative.")
+        out = {k: np.zeros(self._S, np.float32) for k in ("site_mean", "site_min", "site_max")}
+        if self._S:
+            check(lib().gsmc_run_fb_summary(self._h, _ptr(pi), _ptr(pj), P, int(core_block_sites), int(flank_sites),
+                                            _ptr(out["site_mean"]), _ptr(out["site_min"]), _ptr(out["site_max"])))
+        out["n_pairs"] = P
+        return out
 
     def checkpoint_bytes(self, n_sites, core_block_sites, flank_sites, n_pairs):
         out = ctypes.c_uint64()

@@ -20,6 +20,7 @@
 // fused multiply-adds are the explicit fmaf() calls below.
 #pragma once
 
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -81,6 +82,8 @@ struct JobDev {
     const int32_t* pair_j;
     int64_t P;
     int64_t row0;
+    int64_t rows;      // sites of the job (rows of its summary partial planes)
+    int64_t part_off;  // summary launches: element offset of the job's [groups][rows] partials
 };
 
 // One block window of a job. The walk starts from the prior at ps; the first
@@ -92,6 +95,8 @@ struct BlockDev {
     int32_t job;
     int32_t nwin;    // checkpoint windows of the core
     int64_t ck_off;  // float2 offset of this block's checkpoints (tile-major)
+    int32_t q0;      // the walk starts from prior[q0]: the prior after q0 full N-bp hom stretches
+    int32_t pad;
 };
 
 // A work item: one tile (64 pairs per lane slot) of one block window.
@@ -115,6 +120,10 @@ struct FbArgs {
     int* queue2;               // persistent mode: item counter of the second kernel
     DevTables t;
     float unscale;             // 2 Ne
+    const float2* prior;       // prior[q] = q full N-bp hom stretches applied to the prior (0, 0)
+    double* psum;              // summary launches: per 64-pair group and site, the sum of the means
+    float* pmin;               // ... their minimum
+    float* pmax;               // ... their maximum
 };
 
 // Bilinear lookup of one table slab at (m, c); updates (m, c). het (0 or
@@ -263,6 +272,7 @@ template <int NP> struct Lanes {
     bool live[NP];      // the pair exists (idle lanes shadow the job's last pair)
     uint32_t ck_col;    // byte offset of the lane in a checkpoint row of 64 NP float2
     float2* ck;         // checkpoint rows of the item: window w at ck + w * 64 NP
+    int32_t grp0;       // 64-pair group of slot q = 0 (groups are tile * NP + q)
 };
 
 template <int K, int NP> struct Window {
@@ -284,6 +294,69 @@ __device__ __forceinline__ float2* ck_at(const Lanes<NP>& L, int64_t w, int q)
     return global_at<float2>(L.ck + (size_t)w * 64 * NP, L.ck_col + (uint32_t)q * 512u);
 }
 
+// Per-site reduction of one window's means over the 64 lanes of each pair
+// group, for summary launches. The means go through LDS (one row of 65
+// floats per site, so both the writes and the reads below are free of bank
+// conflicts); lane (k = lane / 4, p = lane % 4) then sums lanes 16p .. 16p+15
+// of site k in order, in FP64 from 0.0, and a butterfly over p (partner
+// p ^ 1, then p ^ 2) joins the four parts. Lanes without a pair hold 0 and
+// are left out of min / max. The tree is the same for every K, so results
+// do not depend on the checkpoint interval. One lane per site stores the
+// group's partials.
+template <int K, int NP>
+__device__ __forceinline__ void summarize(const FbArgs& a, const JobDev& j, const Lanes<NP>& L, int64_t ws,
+                                          int64_t cnt, const float (&out)[NP][K], float* red)
+{
+    const int lane = (int)threadIdx.x;
+    const int k = lane >> 2, part = lane & 3;
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+        // pairs of this group that exist: a prefix of the 64 lanes
+        const int64_t first = (int64_t)(L.grp0 + q) * 64;
+        const int live = (int)min(max(j.P - first, (int64_t)0), (int64_t)64);
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < K; ++i) red[i * 65 + lane] = L.live[q] ? out[q][i] : 0.0f;
+        __syncthreads();
+        double sum = 0.0;
+        float lo = __builtin_huge_valf(), hi = -__builtin_huge_valf();
+        if (k < K) {
+            const float* row = red + k * 65 + part * 16;
+            if (live == 64) {
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    const float v = row[i];
+                    sum = sum + (double)v;
+                    lo = fminf(lo, v);
+                    hi = fmaxf(hi, v);
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    const float v = row[i];
+                    sum = sum + (double)v;
+                    if (part * 16 + i < live) {
+                        lo = fminf(lo, v);
+                        hi = fmaxf(hi, v);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int mask = 1; mask <= 2; mask <<= 1) {
+            sum = sum + __shfl_xor(sum, mask);
+            lo = fminf(lo, __shfl_xor(lo, mask));
+            hi = fmaxf(hi, __shfl_xor(hi, mask));
+        }
+        if (part == 0 && k < cnt) {
+            const int64_t idx = j.part_off + (int64_t)(L.grp0 + q) * j.rows + (ws + k - j.row0);
+            a.psum[idx] = sum;
+            a.pmin[idx] = lo;
+            a.pmax[idx] = hi;
+        }
+    }
+}
+
 // Phase-3 step over one checkpoint window, software-pipelined: the
 // backward walk of window w (cnt sites at ws, forward states in F) runs in
 // the same loop as the forward recompute of window w - 1 (K sites) when
@@ -295,12 +368,12 @@ __device__ __forceinline__ float2* ck_at(const Lanes<NP>& L, int64_t w, int q)
 // store (vector-memory counters are shared by loads and stores). FAST (a
 // full window and a full next window, no long gaps in either) has no branch
 // inside the loop, so the compiler can interleave all chains.
-template <int K, int NP, bool CI, bool POST, bool B, bool FAST>
+template <int K, int NP, bool CI, bool POST, bool SUM, bool B, bool FAST>
 __device__ __forceinline__ void window_step(const FbArgs& a, const JobDev& j, const Lanes<NP>& L, int64_t ws,
                                             int64_t cnt, bool has_next, const Window<K, NP>& cur,
                                             const Window<K, NP>& nxt, float (&m)[NP], float (&c)[NP],
                                             float (&fm)[NP][K], float (&fc)[NP][K], float (&ym)[NP],
-                                            float (&yc)[NP])
+                                            float (&yc)[NP], float* red)
 {
     float out[NP][K], oa[NP][K], ob[NP][K];
 #pragma unroll
@@ -316,6 +389,9 @@ __device__ __forceinline__ void window_step(const FbArgs& a, const JobDev& j, co
                 else
                     bwd_segment(a.t, cur.d[k], het_bit(cur.bits[q], k), ym[q], yc[q]);
             }
+        } else if (SUM) {
+#pragma unroll
+            for (int q = 0; q < NP; ++q) out[q][k] = 0.0f;
         }
         if (FAST || has_next) {
 #pragma unroll
@@ -332,6 +408,10 @@ __device__ __forceinline__ void window_step(const FbArgs& a, const JobDev& j, co
         // still interleave inside it, but nothing is hoisted from later
         // sites, which would cost registers and so waves.
         if (FAST) __builtin_amdgcn_sched_barrier(0);
+    }
+    if (SUM) {
+        summarize<K, NP>(a, j, L, ws, cnt, out, red);
+        return;
     }
 #pragma unroll
     for (int q = 0; q < NP; ++q) {
@@ -385,6 +465,7 @@ __device__ __forceinline__ ItemView<NP> open_item(const FbArgs& a, int item_in, 
     // Every lane, idle ones included, owns its checkpoint columns.
     v.L.ck = a.ck + (size_t)v.b.ck_off + (size_t)it.tile * (size_t)v.b.nwin * 64 * NP;
     v.L.ck_col = (uint32_t)lane * 8u;
+    v.L.grp0 = it.tile * NP;
     return v;
 }
 
@@ -406,9 +487,16 @@ __device__ __forceinline__ void forward_item(const FbArgs& a, int item, int lane
     const Lanes<NP>& L = v.L;
     const int64_t cs = b.cs, ce = b.ce, ps = b.ps, pe = b.pe;
 
+    // Start state: the prior, advanced by q0 full stretches at a sequence
+    // start (the same lookups the first site's segment would apply, done
+    // once per context instead of once per item).
+    const float2 x0 = load_const(a.prior + b.q0);
     float m[NP], c[NP];
 #pragma unroll
-    for (int q = 0; q < NP; ++q) m[q] = c[q] = 0.0f;
+    for (int q = 0; q < NP; ++q) {
+        m[q] = x0.x;
+        c[q] = x0.y;
+    }
     for (int64_t s = ps; s < ce;) {
         const int64_t e = (s < cs) ? min(cs, s + K) : min(ce, s + K);
         if (s >= cs) {
@@ -461,8 +549,8 @@ __device__ __forceinline__ void forward_item(const FbArgs& a, int item, int lane
 // Second half of an item: the core windows, last to first. The last window
 // (possibly partial) is recomputed alone into orientation A; every later
 // recompute rides along the previous window's backward walk.
-template <int K, int NP, bool CI, bool POST>
-__device__ __forceinline__ void backward_item(const FbArgs& a, int item, int lane)
+template <int K, int NP, bool CI, bool POST, bool SUM>
+__device__ __forceinline__ void backward_item(const FbArgs& a, int item, int lane, float* red)
 {
     const ItemView<NP> v = open_item<NP>(a, item, lane);
     if (!v.valid) return;
@@ -529,14 +617,14 @@ __device__ __forceinline__ void backward_item(const FbArgs& a, int item, int lan
         const bool fast = has_next && cnt == K && no_long_gaps<K>(cur.d) && no_long_gaps<K>(nxt.d);
         if (fast) {
             if (orient_b)
-                window_step<K, NP, CI, POST, true, true>(a, j, L, ws, cnt, true, cur, nxt, m, c, fm, fc, ym, yc);
+                window_step<K, NP, CI, POST, SUM, true, true>(a, j, L, ws, cnt, true, cur, nxt, m, c, fm, fc, ym, yc, red);
             else
-                window_step<K, NP, CI, POST, false, true>(a, j, L, ws, cnt, true, cur, nxt, m, c, fm, fc, ym, yc);
+                window_step<K, NP, CI, POST, SUM, false, true>(a, j, L, ws, cnt, true, cur, nxt, m, c, fm, fc, ym, yc, red);
         } else {
             if (orient_b)
-                window_step<K, NP, CI, POST, true, false>(a, j, L, ws, cnt, has_next, cur, nxt, m, c, fm, fc, ym, yc);
+                window_step<K, NP, CI, POST, SUM, true, false>(a, j, L, ws, cnt, has_next, cur, nxt, m, c, fm, fc, ym, yc, red);
             else
-                window_step<K, NP, CI, POST, false, false>(a, j, L, ws, cnt, has_next, cur, nxt, m, c, fm, fc, ym, yc);
+                window_step<K, NP, CI, POST, SUM, false, false>(a, j, L, ws, cnt, has_next, cur, nxt, m, c, fm, fc, ym, yc, red);
         }
         if (!has_next) break;
         orient_b = !orient_b;
@@ -558,11 +646,12 @@ __global__ __launch_bounds__(64) void fb_fwd_items(FbArgs a)
     if (item < a.n_items) forward_item<K, NP>(a, item, threadIdx.x);
 }
 
-template <int K, int NP, bool CI, bool POST>
+template <int K, int NP, bool CI, bool POST, bool SUM>
 __global__ __launch_bounds__(64) void fb_bwd_items(FbArgs a)
 {
     const int item = __builtin_amdgcn_readfirstlane((int)blockIdx.x);
-    if (item < a.n_items) backward_item<K, NP, CI, POST>(a, item, threadIdx.x);
+    __shared__ float red[SUM ? 16 * 65 : 1];
+    if (item < a.n_items) backward_item<K, NP, CI, POST, SUM>(a, item, threadIdx.x, red);
 }
 
 __device__ __forceinline__ int next_item(int* counter, int lane)
@@ -579,11 +668,171 @@ __global__ __launch_bounds__(64) void fb_fwd_queue(FbArgs a)
         forward_item<K, NP>(a, item, threadIdx.x);
 }
 
-template <int K, int NP, bool CI, bool POST>
+template <int K, int NP, bool CI, bool POST, bool SUM>
 __global__ __launch_bounds__(64) void fb_bwd_queue(FbArgs a)
 {
+    __shared__ float red[SUM ? 16 * 65 : 1];
     for (int item = next_item(a.queue2, threadIdx.x); item < a.n_items; item = next_item(a.queue2, threadIdx.x))
-        backward_item<K, NP, CI, POST>(a, item, threadIdx.x);
+        backward_item<K, NP, CI, POST, SUM>(a, item, threadIdx.x, red);
+}
+
+// Summary epilogue: per site r of a job, the groups' partials in group
+// order -- the FP64 sum g = 0, 1, ... (starting from the running total of
+// earlier pair chunks unless `first`), min and max. The last chunk writes
+// mean = (float)(sum / n_pairs), min and max; earlier chunks keep the
+// running values in tot / tmin / tmax.
+__global__ __launch_bounds__(256) void summary_reduce_kernel(const double* __restrict__ psum,
+                                                             const float* __restrict__ pmin,
+                                                             const float* __restrict__ pmax, int64_t groups,
+                                                             int64_t rows, double* tot, float* tmin, float* tmax,
+                                                             int first, int last, double n_pairs, float* site_mean,
+                                                             float* site_min, float* site_max)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    double t = first ? 0.0 : tot[r];
+    float lo = first ? __builtin_huge_valf() : tmin[r];
+    float hi = first ? -__builtin_huge_valf() : tmax[r];
+    for (int64_t g = 0; g < groups; ++g) {
+        t = t + psum[g * rows + r];
+        lo = fminf(lo, pmin[g * rows + r]);
+        hi = fmaxf(hi, pmax[g * rows + r]);
+    }
+    if (last) {
+        site_mean[r] = (float)(t / n_pairs);
+        if (site_min) site_min[r] = lo;
+        if (site_max) site_max[r] = hi;
+    } else {
+        tot[r] = t;
+        tmin[r] = lo;
+        tmax[r] = hi;
+    }
+}
+
+// prior[q] for q < n: q applications of the N-bp hom stretch to the prior
+// (0, 0), one dependent lookup each, with the decode kernels' own lookup.
+__global__ void prior_table_kernel(DevTables t, float2* out, int64_t n)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    float m = 0.0f, c = 0.0f;
+    out[0] = make_float2(m, c);
+    for (int64_t q = 1; q < n; ++q) {
+        lookup(t.stretch, 0u, m, c);
+        out[q] = make_float2(m, c);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Forward-only filter (run_fwd): the reference's half-precision cached path,
+// gamma_smc_flow.cu:1139-1232. Tables: the composed transitions cache[k]
+// (N layers of 2550 (m, c) nodes) stored as FP16 pairs. Per site: advance by
+// the site's gap (descriptor kNoAdvance at site 0 and for gaps that round
+// to 0; full N-bp layers first), apply the het emission explicitly, output
+// 10^m * 2Ne (and the Wilson-Hilferty bounds). One lane per pair; outputs of
+// each 16-site window are buffered and stored after it.
+constexpr uint32_t kNoAdvance = 0xFFFFFFFFu;
+
+struct FwdArgs {
+    const uint64_t* packed;
+    int64_t n_haps;
+    int64_t W;
+    int64_t S;
+    const uint32_t* desc;   // [S + 64] forward-only gap descriptors
+    const __half2* h2;      // [N][2550] (m, c)
+    int32_t N;
+    const int32_t* pair_i;
+    const int32_t* pair_j;
+    int64_t P;
+    float two_ne;
+    float* mean;            // [S][P]
+    float* lower;
+    float* upper;
+};
+
+__device__ __forceinline__ void lookup_h2(const __half2* __restrict__ layer, float& m, float& c)
+{
+    float fm = (m - -5.0f) * kMeanStepInv;
+    float fc = (c - -2.0f) * kCvStepInv;
+    fm = __builtin_amdgcn_fmed3f(fm, 0.0f, 50.0f);
+    fc = __builtin_amdgcn_fmed3f(fc, 0.0f, 49.0f);
+    const float lm = fminf(floorf(fm), 49.0f), lc = fminf(floorf(fc), 48.0f);
+    const float wm = fm - lm, wc = fc - lc;
+    const int base = (int)lm * 50 + (int)lc;
+    const float2 v00 = __half22float2(layer[base]), v01 = __half22float2(layer[base + 1]);
+    const float2 v10 = __half22float2(layer[base + 50]), v11 = __half22float2(layer[base + 51]);
+    const float omwm = 1.0f - wm, omwc = 1.0f - wc;
+    m = fmaf(wm, fmaf(wc, v11.x, omwc * v10.x), omwm * fmaf(wc, v01.x, omwc * v00.x));
+    c = fmaf(wm, fmaf(wc, v11.y, omwc * v10.y), omwm * fmaf(wc, v01.y, omwc * v00.y));
+}
+
+template <bool CI>
+__global__ __launch_bounds__(64) void fwd_only_kernel(FwdArgs a)
+{
+    const int64_t p = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    const bool live = p < a.P;
+    const int64_t pp = live ? p : a.P - 1;
+    const int64_t hi = min(max((int64_t)a.pair_i[pp], (int64_t)0), a.n_haps - 1);
+    const int64_t hj = min(max((int64_t)a.pair_j[pp], (int64_t)0), a.n_haps - 1);
+    const uint64_t* ri = a.packed + (size_t)hi * (size_t)a.W;
+    const uint64_t* rj = a.packed + (size_t)hj * (size_t)a.W;
+    const __half2* last = a.h2 + (size_t)(a.N - 1) * kGridNodes;
+    float m = 0.0f, c = 0.0f;
+    for (int64_t s0 = 0; s0 < a.S; s0 += 16) {
+        uint32_t d[16];
+        load_descs<16>(a.desc, s0, d);
+        const uint64_t bits = het_mask(ri, rj, s0, a.W);
+        const int64_t cnt = min((int64_t)16, a.S - s0);
+        float out[16], olo[16], ohi[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            if (k < cnt) {
+                const uint32_t dk = d[k];
+                if (dk != kNoAdvance) {
+                    for (uint32_t q = dk >> kDescLayerBits; q != 0; --q) lookup_h2(last, m, c);
+                    lookup_h2(a.h2 + (size_t)(dk & kDescLayerMask) * kGridNodes, m, c);
+                }
+                if (het_bit(bits, k)) {
+                    const float alpha = exp10f(-2.0f * c) + 1.0f;
+                    const float a_log = log10f(alpha);
+                    const float b_log = fmaf(-2.0f, c, -m);
+                    m = a_log - b_log;
+                    c = -0.5f * a_log;
+                }
+                const float mg = exp2f(m * 3.321928094887362f) * a.two_ne;
+                out[k] = mg;
+                if (CI) {
+                    const float alpha = exp10f(-2.0f * c);
+                    const float inv9a = 1.0f / (9.0f * alpha);
+                    const float sq = sqrtf(inv9a);
+                    const float base = 1.0f - inv9a;
+                    const float lo = fmaxf(fmaf(-1.96f, sq, base), 0.0f);
+                    const float hf = fmaf(1.96f, sq, base);
+                    olo[k] = fmaxf(mg * lo * lo * lo, 0.0f);
+                    ohi[k] = mg * hf * hf * hf;
+                }
+            }
+        }
+        if (live) {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                if (k < cnt) {
+                    const size_t idx = (size_t)(s0 + k) * (size_t)a.P + (size_t)p;
+                    __builtin_nontemporal_store(out[k], a.mean + idx);
+                    if (CI) {
+                        __builtin_nontemporal_store(olo[k], a.lower + idx);
+                        __builtin_nontemporal_store(ohi[k], a.upper + idx);
+                    }
+                }
+            }
+        }
+    }
+}
+
+// FP32 (m, c) pairs to FP16 pairs, round to nearest even.
+__global__ void to_half2_kernel(const float2* __restrict__ src, __half2* __restrict__ dst, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = __floats2half2_rn(src[i].x, src[i].y);
 }
 
 __global__ void reset_counters_kernel(int* counters)

@@ -127,12 +127,12 @@ struct DeviceTables {
     char* base = nullptr;  // fwd layers, bwd layers, stretch slab
     size_t bytes = 0;
     DevTables view{};
+    __half2* h2 = nullptr;  // forward-only path: cache[k] as FP16 (m, c), [N][2550]
     ~DeviceTables()
     {
-        if (base) {
-            (void)hipSetDevice(device);
-            (void)hipFree(base);
-        }
+        if (base || h2) (void)hipSetDevice(device);
+        if (base) (void)hipFree(base);
+        if (h2) (void)hipFree(h2);
     }
 };
 
@@ -193,6 +193,20 @@ std::shared_ptr<DeviceTables> get_tables(int device, const char* path, int N, do
     dt->view.fwd = dt->base;
     dt->view.bwd = dt->base + layers;
     dt->view.stretch = dt->base + 2 * layers;
+    {
+        // FP16 copy of the composed transitions, rounded on the device as
+        // the reference rounds them on the host (__floats2half2_rn).
+        const int64_t nodes = (int64_t)N * kGridNodes;
+        float2* tmp = nullptr;
+        HIP_OK(hipMalloc(&tmp, (size_t)nodes * sizeof(float2)));
+        HIP_OK(hipMalloc(&dt->h2, (size_t)nodes * sizeof(__half2)));
+        HIP_OK(hipMemcpy(tmp, host.cache.data(), (size_t)nodes * sizeof(float2), hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(to_half2_kernel, dim3((unsigned)((nodes + 255) / 256)), dim3(256), 0, nullptr, tmp, dt->h2,
+                           nodes);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipDeviceSynchronize());
+        HIP_OK(hipFree(tmp));
+    }
     g_tabs[key] = dt;
     for (auto i = g_tabs.begin(); i != g_tabs.end();) i = i->second.expired() ? g_tabs.erase(i) : std::next(i);
     return dt;
@@ -210,6 +224,19 @@ int round_steps(double delta)
     else v = (int)r;
     return v > 1 ? v : 1;  // rounded_segment_steps (gamma_smc_flow.cu:160-163)
 }
+
+// (int)(x) as the device computes it: truncate, saturating, NaN -> 0.
+int trunc_steps(double x)
+{
+    if (x != x) return 0;
+    if (x >= 2147483647.0) return 2147483647;
+    if (x <= -2147483648.0) return INT32_MIN;
+    return (int)x;
+}
+
+// Prior-after-stretches table bound: 4M entries (32 MB), i.e. positions up
+// to 4e9 bp at the default 1000-step tables.
+constexpr int64_t kMaxPriorStretches = 1 << 22;
 
 uint32_t descriptor(int steps, int N)
 {
@@ -245,6 +272,10 @@ struct gsmc_context {
     Tuning tune;
     std::vector<double> pos;
     DevBuf packed, desc;
+    DevBuf prior;              // prior[q]: the prior after q full stretches (sequence starts)
+    int64_t prior_n = 0;
+    DevBuf desc_h2;            // forward-only path: per-site gap descriptors
+    DevBuf fwd_buf;            // forward-only host path: pair chunk and outputs
     std::shared_ptr<DeviceTables> tabs;
     // host-output path
     DevBuf pairs;
@@ -256,6 +287,7 @@ struct gsmc_context {
     hipEvent_t ev_stage[2] = {nullptr, nullptr};   // staging buffer filled (copy -> host)
     std::unique_ptr<gsmc_batch> host_batch[2];
     std::unique_ptr<gsmc_batch> dev_batch;
+    DevBuf sum_pairs, sum_site;  // per-site summary path: pair chunk, outputs and running values
 
     ~gsmc_context();
 };
@@ -265,7 +297,7 @@ struct gsmc_context {
 // unless it repeats the same metadata on the same stream.
 struct gsmc_batch {
     gsmc_context* ctx = nullptr;
-    DevBuf meta, ck, queue, yb;
+    DevBuf meta, ck, queue, yb, part;
     PinnedBuf stage;
     std::vector<char> last;
     hipStream_t last_stream = nullptr;
@@ -336,6 +368,26 @@ void finish_create(gsmc_context* ctx, const double* positions, double Ne, double
     }
     ctx->desc.ensure(d.size() * 4, ctx->device);
     HIP_OK(hipMemcpy(ctx->desc.p, d.data(), d.size() * 4, hipMemcpyHostToDevice));
+    // Forward-only path (gamma_smc_flow.cu:1212-1217): gap = (int)(delta + 0.5)
+    // from the previous site, nothing at site 0 or for gaps that round to 0.
+    for (int64_t s = 0; s < ctx->S; ++s) {
+        const int gap = s == 0 ? 0 : trunc_steps(positions[s] - positions[s - 1] + 0.5);
+        d[s] = gap > 0 ? descriptor(gap, ctx->N) : kNoAdvance;
+    }
+    ctx->desc_h2.ensure(d.size() * 4, ctx->device);
+    HIP_OK(hipMemcpy(ctx->desc_h2.p, d.data(), d.size() * 4, hipMemcpyHostToDevice));
+    // A sequence may start at any site; its first segment applies
+    // (steps - 1) / N full stretches to the prior. Precompute the prior after
+    // q stretches for every q any site can need (bounded; beyond the bound
+    // the kernels apply the stretches themselves).
+    int64_t qmax = 0;
+    for (int64_t s = 0; s < ctx->S; ++s) qmax = std::max<int64_t>(qmax, (round_steps(positions[s] + 1.0) - 1) / ctx->N);
+    ctx->prior_n = std::min<int64_t>(qmax, kMaxPriorStretches) + 1;
+    ctx->prior.ensure((size_t)ctx->prior_n * sizeof(float2), ctx->device);
+    hipLaunchKernelGGL(prior_table_kernel, dim3(1), dim3(64), 0, nullptr, ctx->tabs->view, ctx->prior.as<float2>(),
+                       ctx->prior_n);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipDeviceSynchronize());
     hipDeviceProp_t prop;
     HIP_OK(hipGetDeviceProperties(&prop, ctx->device));
     ctx->cus = prop.multiProcessorCount;
@@ -362,6 +414,9 @@ struct Outputs {
     float* upper;
     float* alpha;
     float* beta;
+    float* site_mean;  // summary launches: per-site reductions over the pairs
+    float* site_min;
+    float* site_max;
 };
 
 // One job of a launch: pairs (device ids) decoded over `blocks` (global site
@@ -375,6 +430,19 @@ struct JobSpec {
     int64_t P = 0;
     Outputs out{};
     int64_t row0 = 0;
+    // Summary launches over pair chunks: the first chunk starts the running
+    // per-site values, the last one writes the outputs (over P_total pairs).
+    bool sum_first = true, sum_last = true;
+    double* tot = nullptr;
+    float* tmin = nullptr;
+    float* tmax = nullptr;
+    int64_t P_total = 0;
+};
+
+// Summary partials of one job in a launch.
+struct SumJob {
+    const JobSpec* spec;
+    int64_t part_off, groups, rows;
 };
 
 // Launch metadata: [jobs][blocks][items], 256-byte aligned sections.
@@ -384,32 +452,57 @@ struct Plan {
     int64_t n_items = 0;
     int64_t n_waves = 0;  // waves with work (items without padding)
     size_t ck_elems = 0;
+    size_t part_elems = 0;        // summary partials (per plane)
+    std::vector<SumJob> sums;
 };
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-Plan make_plan(const gsmc_context* ctx, const std::vector<JobSpec>& jobs, const Tuning& tu, int np)
+Plan make_plan(const gsmc_context* ctx, const std::vector<JobSpec>& jobs, const Tuning& tu, int np,
+               bool summary)
 {
     std::vector<JobDev> jd;
     std::vector<BlockDev> bd;
     std::vector<ItemDev> items;
-    size_t ck = 0;
+    std::vector<SumJob> sums;
+    size_t ck = 0, part = 0;
     for (size_t j = 0; j < jobs.size(); ++j) {
         const JobSpec& js = jobs[j];
         if (js.P == 0 || js.blocks.empty()) continue;
-        jd.push_back({js.out.mean, js.out.lower, js.out.upper, js.out.alpha, js.out.beta, js.pi, js.pj, js.P,
-                      js.row0});
         const int64_t tile_pairs = 64 * (int64_t)np;
         const int64_t tiles = (js.P + tile_pairs - 1) / tile_pairs;
+        const int64_t rows = (int64_t)js.blocks.back().ce - js.row0;
+        int64_t part_off = -1;
+        if (summary) {
+            part_off = (int64_t)part;
+            sums.push_back({&js, part_off, tiles * np, rows});
+            part += (size_t)(tiles * np) * (size_t)rows;
+        }
+        jd.push_back({js.out.mean, js.out.lower, js.out.upper, js.out.alpha, js.out.beta, js.pi, js.pj, js.P,
+                      js.row0, rows, part_off});
         for (const Block& b : js.blocks) {
             BlockDev d;
             d.cs = b.cs;
             d.ce = b.ce;
             d.ps = b.ps;
             d.pe = b.pe;
+            d.q0 = 0;
+            d.pad = 0;
             if (b.ps == js.seq_lo) {
-                // first site of the decoded sequence: its gap from position -1
-                d.sd_fwd = d.sd_bwd = descriptor(round_steps(ctx->pos[b.ps] + 1.0), ctx->N);
+                // first site of the decoded sequence: its gap from position -1.
+                // The full stretches of that gap start from the prior, so the
+                // walk starts at prior[q] and applies only the last layer.
+                // The backward walk's step at this site comes after the
+                // site's combine and is never used: one lookup stands in.
+                const uint32_t full = descriptor(round_steps(ctx->pos[b.ps] + 1.0), ctx->N);
+                const int64_t q = full >> kDescLayerBits;
+                if (q < ctx->prior_n) {
+                    d.sd_fwd = full & kDescLayerMask;
+                    d.q0 = (int32_t)q;
+                } else {
+                    d.sd_fwd = full;
+                }
+                d.sd_bwd = 0;
             } else {
                 // block restart: one bp from (position - 1), as the block kernels do
                 const double pp = ctx->pos[b.ps];
@@ -427,6 +520,17 @@ Plan make_plan(const gsmc_context* ctx, const std::vector<JobSpec>& jobs, const 
     }
     Plan pl;
     pl.n_waves = (int64_t)items.size();
+    // Longest chains first: an item's duration is its walk length (forward
+    // over [ps, ce), backward over the right flank, two lookups per core
+    // site), so ragged batches dispatch their long items before the short
+    // ones that fill in around them. Items of equal length keep their order.
+    std::stable_sort(items.begin(), items.end(), [&](const ItemDev& x, const ItemDev& y) {
+        auto cost = [&](const ItemDev& it) {
+            const BlockDev& b = bd[(size_t)it.block];
+            return (int64_t)(b.pe - b.ps) + 2 * (int64_t)(b.ce - b.cs);
+        };
+        return cost(x) > cost(y);
+    });
     if (tu.schedule == 0 && tu.xcd_order && !items.empty()) {
         // Workgroup g runs on XCD g % 8: hand each XCD a contiguous run of
         // workgroup-sized chunks, so the waves sharing an XCD's L2 walk the
@@ -446,6 +550,8 @@ Plan make_plan(const gsmc_context* ctx, const std::vector<JobSpec>& jobs, const 
     }
     pl.n_items = (int64_t)items.size();
     pl.ck_elems = ck;
+    pl.part_elems = part;
+    pl.sums = std::move(sums);
     const size_t sj = jd.size() * sizeof(JobDev), sb = bd.size() * sizeof(BlockDev);
     pl.off_blocks = align256(sj);
     pl.off_items = pl.off_blocks + align256(sb);
@@ -468,17 +574,18 @@ struct KernelPair {
     const void* bwd_queue;
 };
 
-template <int K, int NP, bool C, bool Q> KernelPair kernels_of()
+template <int K, int NP, bool C, bool Q, bool SUM = false> KernelPair kernels_of()
 {
     return {reinterpret_cast<const void*>(&fb_fwd_items<K, NP>),
-            reinterpret_cast<const void*>(&fb_bwd_items<K, NP, C, Q>),
+            reinterpret_cast<const void*>(&fb_bwd_items<K, NP, C, Q, SUM>),
             reinterpret_cast<const void*>(&fb_fwd_queue<K, NP>),
-            reinterpret_cast<const void*>(&fb_bwd_queue<K, NP, C, Q>)};
+            reinterpret_cast<const void*>(&fb_bwd_queue<K, NP, C, Q, SUM>)};
 }
 
 template <int K> KernelPair kernels_k(const Tuning& tu, int flags)
 {
     const bool ci = flags & GSMC_CI, post = flags & GSMC_POSTERIOR;
+    if (flags & GSMC_SUMMARY) return tu.np == 2 ? kernels_of<K, 2, false, false, true>() : kernels_of<K, 1, false, false, true>();
     if (ci && post) return kernels_of<K, 1, true, true>();
     if (ci) return kernels_of<K, 1, true, false>();
     if (post) return kernels_of<K, 1, false, true>();
@@ -509,7 +616,8 @@ void launch_plan(gsmc_batch* b, const std::vector<JobSpec>& jobs, int flags, hip
 {
     gsmc_context* ctx = b->ctx;
     const Tuning tu = ctx->tune;
-    Plan pl = make_plan(ctx, jobs, tu, launch_np(tu, flags));
+    const bool summary = (flags & GSMC_SUMMARY) != 0;
+    Plan pl = make_plan(ctx, jobs, tu, launch_np(tu, flags), summary);
     if (pl.n_items == 0) return;
     const bool same = b->pending && stream == b->last_stream && pl.bytes == b->last;
     if (!same) {
@@ -530,6 +638,7 @@ void launch_plan(gsmc_batch* b, const std::vector<JobSpec>& jobs, int flags, hip
     a.n_haps = ctx->n;
     a.W = ctx->W;
     a.desc = ctx->desc.as<uint32_t>();
+    a.prior = ctx->prior.as<float2>();
     a.jobs = reinterpret_cast<const JobDev*>(b->meta.p);
     a.blocks = reinterpret_cast<const BlockDev*>(b->meta.as<char>() + pl.off_blocks);
     a.items = reinterpret_cast<const ItemDev*>(b->meta.as<char>() + pl.off_items);
@@ -540,6 +649,14 @@ void launch_plan(gsmc_batch* b, const std::vector<JobSpec>& jobs, int flags, hip
     a.yb = b->yb.as<float2>();
     a.t = ctx->tabs->view;
     a.unscale = 2.0f * ctx->Ne;
+    a.psum = nullptr;
+    a.pmin = a.pmax = nullptr;
+    if (summary) {
+        b->part.ensure(std::max<size_t>(pl.part_elems, 1) * (sizeof(double) + 2 * sizeof(float)), ctx->device);
+        a.psum = b->part.as<double>();
+        a.pmin = reinterpret_cast<float*>(a.psum + std::max<size_t>(pl.part_elems, 1));
+        a.pmax = a.pmin + std::max<size_t>(pl.part_elems, 1);
+    }
     const KernelPair kp = kernels_for(tu, flags);
     if (tu.schedule == 1) {
         hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(1), 0, stream, a.queue);
@@ -548,6 +665,14 @@ void launch_plan(gsmc_batch* b, const std::vector<JobSpec>& jobs, int flags, hip
     } else {
         launch(kp.fwd_items, (unsigned)pl.n_items, a, stream);
         launch(kp.bwd_items, (unsigned)pl.n_items, a, stream);
+    }
+    for (const SumJob& sj : pl.sums) {
+        const JobSpec& js = *sj.spec;
+        const unsigned grid = (unsigned)((sj.rows + 255) / 256);
+        hipLaunchKernelGGL(summary_reduce_kernel, dim3(grid), dim3(256), 0, stream, a.psum + sj.part_off,
+                           a.pmin + sj.part_off, a.pmax + sj.part_off, sj.groups, sj.rows, js.tot, js.tmin, js.tmax,
+                           (int)js.sum_first, (int)js.sum_last, (double)(js.P_total > 0 ? js.P_total : js.P),
+                           js.out.site_mean, js.out.site_min, js.out.site_max);
     }
     HIP_OK(hipGetLastError());
     HIP_OK(hipEventRecord(b->done, stream));
@@ -992,6 +1117,158 @@ int gsmc_run_fb_device(gsmc_context* ctx, const int32_t* d_pi, const int32_t* d_
     });
 }
 
+int gsmc_run_fb_summary(gsmc_context* ctx, const int32_t* pi, const int32_t* pj, int64_t P, int64_t core,
+                        int64_t flank, float* site_mean, float* site_min, float* site_max)
+{
+    return guarded([&] {
+        require(ctx != nullptr, "context is NULL");
+        require(P >= 0, "negative n_pairs");
+        require(flank >= 0, "flank_sites must be non-negative.");
+        if (P == 0 || ctx->S == 0) return;
+        require(pi != nullptr && pj != nullptr && site_mean != nullptr, "NULL argument");
+        check_pairs(ctx, pi, pj, P);
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        HIP_OK(hipSetDevice(ctx->device));
+        if (!ctx->dev_batch) ctx->dev_batch = new_batch(ctx);
+        const int64_t S = ctx->S;
+        const int64_t c = core > 0 ? core : S;
+        const int64_t fl = core > 0 ? flank : 0;
+        // Pairs per chunk: partials (16 B per 64-pair group and site) and
+        // checkpoints (8 B per pair every K sites) within a quarter of free memory.
+        size_t free_b = 0, total_b = 0;
+        HIP_OK(hipMemGetInfo(&free_b, &total_b));
+        const double per_pair = 16.0 / 64.0 * (double)S + 8.0 * (double)(S + 2 * fl) / ctx->tune.K + 64.0;
+        int64_t chunk = (int64_t)(0.25 * (double)free_b / per_pair) / 128 * 128;
+        chunk = std::max<int64_t>(chunk, 128);
+        chunk = std::min<int64_t>(chunk, (P + 127) / 128 * 128);
+        ctx->sum_pairs.ensure((size_t)chunk * 2 * sizeof(int32_t), ctx->device);
+        ctx->sum_site.ensure((size_t)S * (3 * sizeof(float) + sizeof(double) + 2 * sizeof(float)), ctx->device);
+        int32_t* d_pi = ctx->sum_pairs.as<int32_t>();
+        int32_t* d_pj = d_pi + chunk;
+        double* tot = ctx->sum_site.as<double>();
+        float* d_mean = reinterpret_cast<float*>(tot + S);
+        float* d_min = d_mean + S;
+        float* d_max = d_min + S;
+        float* tmin = d_max + S;
+        float* tmax = tmin + S;
+        for (int64_t p0 = 0; p0 < P; p0 += chunk) {
+            const int64_t n = std::min(chunk, P - p0);
+            // the previous chunk's kernels read the pair buffer
+            HIP_OK(hipStreamSynchronize(ctx->compute));
+            HIP_OK(hipMemcpyAsync(d_pi, pi + p0, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, ctx->compute));
+            HIP_OK(hipMemcpyAsync(d_pj, pj + p0, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, ctx->compute));
+            JobSpec js;
+            js.blocks = make_blocks(S, c, fl);
+            js.pi = d_pi;
+            js.pj = d_pj;
+            js.P = n;
+            js.out.site_mean = d_mean;
+            js.out.site_min = d_min;
+            js.out.site_max = d_max;
+            js.sum_first = p0 == 0;
+            js.sum_last = p0 + n == P;
+            js.tot = tot;
+            js.tmin = tmin;
+            js.tmax = tmax;
+            js.P_total = P;
+            launch_plan(ctx->dev_batch.get(), {js}, GSMC_SUMMARY, ctx->compute);
+        }
+        HIP_OK(hipMemcpyAsync(site_mean, d_mean, (size_t)S * sizeof(float), hipMemcpyDeviceToHost, ctx->compute));
+        if (site_min)
+            HIP_OK(hipMemcpyAsync(site_min, d_min, (size_t)S * sizeof(float), hipMemcpyDeviceToHost, ctx->compute));
+        if (site_max)
+            HIP_OK(hipMemcpyAsync(site_max, d_max, (size_t)S * sizeof(float), hipMemcpyDeviceToHost, ctx->compute));
+        HIP_OK(hipStreamSynchronize(ctx->compute));
+    });
+}
+
+namespace gsmc {
+namespace {
+void launch_fwd_only(gsmc_context* ctx, const int32_t* d_pi, const int32_t* d_pj, int64_t P, int flags,
+                     float* d_mean, float* d_lower, float* d_upper, hipStream_t stream)
+{
+    FwdArgs a;
+    a.packed = ctx->packed.as<uint64_t>();
+    a.n_haps = ctx->n;
+    a.W = ctx->W;
+    a.S = ctx->S;
+    a.desc = ctx->desc_h2.as<uint32_t>();
+    a.h2 = ctx->tabs->h2;
+    a.N = ctx->N;
+    a.pair_i = d_pi;
+    a.pair_j = d_pj;
+    a.P = P;
+    a.two_ne = 2.0f * ctx->Ne;
+    a.mean = d_mean;
+    a.lower = d_lower;
+    a.upper = d_upper;
+    const dim3 grid((unsigned)((P + 63) / 64));
+    if (flags & GSMC_CI)
+        hipLaunchKernelGGL(fwd_only_kernel<true>, grid, dim3(64), 0, stream, a);
+    else
+        hipLaunchKernelGGL(fwd_only_kernel<false>, grid, dim3(64), 0, stream, a);
+    HIP_OK(hipGetLastError());
+}
+}  // namespace
+}  // namespace gsmc
+
+int gsmc_run_fwd_device(gsmc_context* ctx, const int32_t* d_pi, const int32_t* d_pj, int64_t P, int flags,
+                        float* d_mean, float* d_lower, float* d_upper, void* stream)
+{
+    return guarded([&] {
+        require(ctx != nullptr, "context is NULL");
+        require(P >= 0, "negative n_pairs");
+        if (P == 0 || ctx->S == 0) return;
+        require(d_pi != nullptr && d_pj != nullptr && d_mean != nullptr, "NULL argument");
+        require(!(flags & GSMC_CI) || (d_lower && d_upper), "lower/upper required with GSMC_CI");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        HIP_OK(hipSetDevice(ctx->device));
+        launch_fwd_only(ctx, d_pi, d_pj, P, flags, d_mean, d_lower, d_upper, static_cast<hipStream_t>(stream));
+    });
+}
+
+int gsmc_run_fwd(gsmc_context* ctx, const int32_t* pi, const int32_t* pj, int64_t P, int flags, float* mean,
+                 float* lower, float* upper)
+{
+    return guarded([&] {
+        require(ctx != nullptr, "context is NULL");
+        require(P >= 0, "negative n_pairs");
+        if (P == 0 || ctx->S == 0) return;
+        require(pi != nullptr && pj != nullptr && mean != nullptr, "NULL argument");
+        const bool ci = (flags & GSMC_CI) != 0;
+        require(!ci || (lower && upper), "lower/upper required with GSMC_CI");
+        check_pairs(ctx, pi, pj, P);
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        HIP_OK(hipSetDevice(ctx->device));
+        const int64_t S = ctx->S;
+        const int nout = ci ? 3 : 1;
+        // Pairs per chunk: outputs of a chunk within a quarter of free memory.
+        size_t free_b = 0, total_b = 0;
+        HIP_OK(hipMemGetInfo(&free_b, &total_b));
+        const double per_pair = 4.0 * nout * (double)S + 8.0;
+        int64_t chunk = (int64_t)(0.25 * (double)free_b / per_pair) / 64 * 64;
+        chunk = std::min<int64_t>(std::max<int64_t>(chunk, 64), (P + 63) / 64 * 64);
+        ctx->fwd_buf.ensure((size_t)chunk * 8 + (size_t)nout * (size_t)S * (size_t)chunk * 4, ctx->device);
+        int32_t* d_pi = ctx->fwd_buf.as<int32_t>();
+        int32_t* d_pj = d_pi + chunk;
+        float* d_out = reinterpret_cast<float*>(d_pj + chunk);
+        float* host[3] = {mean, lower, upper};
+        for (int64_t p0 = 0; p0 < P; p0 += chunk) {
+            const int64_t n = std::min(chunk, P - p0);
+            HIP_OK(hipStreamSynchronize(ctx->compute));
+            HIP_OK(hipMemcpyAsync(d_pi, pi + p0, (size_t)n * 4, hipMemcpyHostToDevice, ctx->compute));
+            HIP_OK(hipMemcpyAsync(d_pj, pj + p0, (size_t)n * 4, hipMemcpyHostToDevice, ctx->compute));
+            float* dev[3] = {d_out, d_out + (size_t)S * n, d_out + 2 * (size_t)S * n};
+            launch_fwd_only(ctx, d_pi, d_pj, n, flags, dev[0], ci ? dev[1] : nullptr, ci ? dev[2] : nullptr,
+                            ctx->compute);
+            for (int o = 0; o < nout; ++o)
+                HIP_OK(hipMemcpy2DAsync(host[o] + p0, (size_t)P * 4, dev[o], (size_t)n * 4, (size_t)n * 4, (size_t)S,
+                                        hipMemcpyDeviceToHost, ctx->compute));
+        }
+        HIP_OK(hipStreamSynchronize(ctx->compute));
+    });
+}
+
 int gsmc_batch_create(gsmc_context* ctx, gsmc_batch** out)
 {
     return guarded([&] {
@@ -1029,6 +1306,8 @@ int gsmc_batch_launch(gsmc_batch* b, const gsmc_job* jobs, int64_t n_jobs, int f
         require(b != nullptr, "batch is NULL");
         require(n_jobs >= 0 && (jobs != nullptr || n_jobs == 0), "bad job list");
         gsmc_context* ctx = b->ctx;
+        require(!((flags & GSMC_SUMMARY) && (flags & (GSMC_CI | GSMC_POSTERIOR))),
+                "GSMC_SUMMARY cannot be combined with GSMC_CI or GSMC_POSTERIOR");
         std::vector<JobSpec> specs;
         specs.reserve((size_t)n_jobs);
         for (int64_t i = 0; i < n_jobs; ++i) {
@@ -1037,7 +1316,8 @@ int gsmc_batch_launch(gsmc_batch* b, const gsmc_job* jobs, int64_t n_jobs, int f
             require(j.n_pairs >= 0, "negative n_pairs");
             require(j.flank_sites >= 0, "flank_sites must be non-negative.");
             if (j.n_pairs == 0 || j.site_hi == j.site_lo) continue;
-            require(j.mean != nullptr && j.pair_i != nullptr && j.pair_j != nullptr, "job pointers are NULL");
+            require(j.pair_i != nullptr && j.pair_j != nullptr, "job pointers are NULL");
+            require((flags & GSMC_SUMMARY) ? j.site_mean != nullptr : j.mean != nullptr, "job outputs are NULL");
             require(!(flags & GSMC_CI) || (j.lower && j.upper), "lower/upper required with GSMC_CI");
             require(!(flags & GSMC_POSTERIOR) || (j.alpha && j.beta), "alpha/beta required with GSMC_POSTERIOR");
             const int64_t len = j.site_hi - j.site_lo;
@@ -1054,7 +1334,7 @@ int gsmc_batch_launch(gsmc_batch* b, const gsmc_job* jobs, int64_t n_jobs, int f
             js.pi = j.pair_i;
             js.pj = j.pair_j;
             js.P = j.n_pairs;
-            js.out = Outputs{j.mean, j.lower, j.upper, j.alpha, j.beta};
+            js.out = Outputs{j.mean, j.lower, j.upper, j.alpha, j.beta, j.site_mean, j.site_min, j.site_max};
             js.row0 = j.site_lo;
             specs.push_back(std::move(js));
         }

@@ -224,6 +224,11 @@ void build(const FlowField& ff, int N, Real rho, Real mu, HostTables& out)
         out.stretch_last[2 * g] = static_cast<float>(sm[(size_t)(N - 1) * kGrid + g]);
         out.stretch_last[2 * g + 1] = static_cast<float>(sc[(size_t)(N - 1) * kGrid + g]);
     }
+    out.cache.resize((size_t)N * kGrid * 2);
+    for (size_t i = 0; i < (size_t)N * kGrid; ++i) {
+        out.cache[2 * i] = static_cast<float>(sm[i]);
+        out.cache[2 * i + 1] = static_cast<float>(sc[i]);
+    }
 }
 
 }  // namespace

@@ -23,6 +23,7 @@ struct HostTables {
     std::vector<float> fwd;           // [N][2 (hom, het)][kGrid] x (m, c)
     std::vector<float> bwd;           // [N][2 (hom, het)][kGrid] x (m, c)
     std::vector<float> stretch_last;  // [kGrid] x (m, c): N-bp hom stretch
+    std::vector<float> cache;         // [N][kGrid] x (m, c): composed k+1-bp transition (forward-only path)
 };
 
 // Parse the text format of src/flow_field.cpp:8-54 (two header lines, then

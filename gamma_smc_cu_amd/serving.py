@@ -13,10 +13,11 @@ Continuous batching.
   stream and launch workspace.
 - Whenever a slot is free it admits waiting queries, oldest first, while they
   fit the HBM budget:
-  - pair ids and outputs go into a device arena allocated once, so the
-    request path never allocates device memory;
-  - checkpoints go into the slot's workspace, reserved up front for the
-    per-launch caps on checkpoint bytes and work items.
+  - outputs go into a device arena allocated once, so the request path
+    never allocates device memory;
+  - checkpoints and the launch's pair ids go into the slot's own buffers,
+    reserved up front for the per-launch caps on checkpoint bytes and work
+    items (the pair ids of a whole launch cross PCIe in one pinned copy).
 - All queries admitted together run as one batched launch. Queries arriving
   while launches run wait for the next free slot and for memory. Launches on
   different slots overlap on the device.
@@ -129,10 +130,6 @@ class Query:
     def latency(self):
         return self.finished - self.arrival
 
-    def _layout(self):
-        """Arena layout: pair_i, pair_j (int32 each), then the output."""
-        pb = _align(4 * self.n_pairs)
-        return pb, 2 * pb, 2 * pb + 4 * self.pair_sites
 
 
 def _n_blocks(n_sites, core, flank):
@@ -175,9 +172,13 @@ class _Slot:
         self.ws = Workspace(ctx)
         _core.check(_core.lib().gsmc_batch_reserve(self.ws.handle, int(ck_bytes), int(max_items), int(max_jobs)))
         self.stream = Stream()
+        self.start = Event()
         self.done = Event()
+        # pair ids of one launch: [pair_i of every query | pair_j of every query]
+        self.max_pairs = 64 * int(max_items)
         self.stage = _PinnedHost()
-        self.stage.ensure(min(8 * 64 * int(max_items), 64 << 20))
+        self.stage.ensure(8 * self.max_pairs)
+        self.d_pairs = DeviceArray((2 * self.max_pairs,), np.int32)
         self.queries = []
 
     @property
@@ -188,8 +189,8 @@ class _Slot:
 class Server:
     """Continuous-batching decoder over a resident cohort.
 
-    ``arena_bytes``: device memory for pair ids and outputs of queries in
-    flight and not yet released. ``ck_bytes_per_launch`` /
+    ``arena_bytes``: device memory for the outputs of queries in flight and
+    not yet released. ``ck_bytes_per_launch`` /
     ``max_items_per_launch`` / ``max_queries_per_launch``: caps on one
     launch (checkpoints, work items, queries). The caps bound the memory each
     slot reserves and a launch's duration. ``max_queries_per_launch=1``
@@ -212,6 +213,7 @@ class Server:
         self.completed = {}
         self.launches = 0
         self.max_in_flight = 0  # most launches seen running at once
+        self.launch_log = []    # per finished launch: (queries, pair-sites, work items, device ms)
         self._next_id = 0
         synchronize()
         self._ev0 = Event()
@@ -233,7 +235,7 @@ class Server:
         if q.flank_sites < 0:
             raise ValueError("flank_sites must be non-negative.")
         q.pairs = subset_pairs(h)
-        q._bytes = q._layout()[2]
+        q._bytes = 4 * q.pair_sites
         q._ck = self.ctx.checkpoint_bytes(q.n_sites, q.core_block_sites, q.flank_sites, q.n_pairs)
         q._items = -(-q.n_pairs // 64) * _n_blocks(q.n_sites, q.core_block_sites, q.flank_sites)
         if _align(q._bytes) > self.arena.size or q._ck > self.ck_cap or q._items > self.item_cap:
@@ -255,6 +257,8 @@ class Server:
             _core.check(_core.lib().gsmc_event_query(slot.done.handle, ctypes.byref(flag)))
             if flag.value:
                 t = self._t0 + self._ev0.elapsed_ms(slot.done) / 1e3
+                self.launch_log.append((len(slot.queries), sum(q.pair_sites for q in slot.queries),
+                                        sum(q._items for q in slot.queries), slot.start.elapsed_ms(slot.done)))
                 for q in slot.queries:
                     q.finished = t
                     self.completed[q.qid] = q
@@ -287,7 +291,7 @@ class Server:
         if q.qid not in self.completed:
             raise ValueError("query has not finished")
         out = np.empty((q.n_sites, q.n_pairs), np.float32)
-        src = self.d_arena.offset(q._off + q._layout()[1])
+        src = self.d_arena.offset(q._off)
         _core.check(_core.lib().gsmc_memcpy_dtoh(ctypes.c_void_p(out.ctypes.data), src, out.nbytes))
         return out
 
@@ -315,29 +319,26 @@ class Server:
 
     def _launch(self, slot, batch):
         total = sum(q.n_pairs for q in batch)
-        slot.stage.ensure(8 * total)
+        pi = slot.stage.view(0, total)
+        pj = slot.stage.view(4 * total, total)
         jobs = (_core.Job * len(batch))()
         base = self.d_arena.ptr.value
+        dp = slot.d_pairs.ptr.value
         at = 0
-        lib = _core.lib()
         for k, q in enumerate(batch):
             P = q.n_pairs
-            stage_i = slot.stage.view(4 * at, P)
-            stage_j = slot.stage.view(4 * (at + P), P)
-            stage_i[:] = q.pairs[:, 0]
-            stage_j[:] = q.pairs[:, 1]
-            pb, out_off, _ = q._layout()
-            for dst, src in ((q._off, at), (q._off + pb, at + P)):
-                _core.check(lib.gsmc_memcpy_htod_async(ctypes.c_void_p(base + dst),
-                                                       ctypes.c_void_p(slot.stage.ptr.value + 4 * src),
-                                                       4 * P, slot.stream.handle))
-            at += 2 * P
+            pi[at:at + P] = q.pairs[:, 0]
+            pj[at:at + P] = q.pairs[:, 1]
             j = jobs[k]
             j.site_lo, j.site_hi = int(q.site_lo), int(q.site_hi)
             j.core_block_sites, j.flank_sites = int(q.core_block_sites), int(q.flank_sites)
-            j.pair_i, j.pair_j = base + q._off, base + q._off + pb
+            j.pair_i, j.pair_j = dp + 4 * at, dp + 4 * (total + at)
             j.n_pairs = P
-            j.mean = base + q._off + out_off
+            j.mean = base + q._off
+            at += P
+        _core.check(_core.lib().gsmc_memcpy_htod_async(slot.d_pairs.ptr, slot.stage.ptr, 8 * total,
+                                                       slot.stream.handle))
+        slot.start.record(slot.stream)
         slot.ws.launch(jobs, len(batch), 0, slot.stream)
         slot.done.record(slot.stream)
         now = self.clock()

@@ -29,6 +29,8 @@ extern "C" {
 /* run flags */
 #define GSMC_CI 1          /* also write Wilson-Hilferty lower / upper (mean_only=False)     */
 #define GSMC_POSTERIOR 2   /* also write posterior alpha / beta (return_posterior=True)       */
+#define GSMC_SUMMARY 4     /* batch jobs: write per-site mean / min / max over the pairs
+                              (site_mean, site_min, site_max) instead of per-pair rows       */
 
 /* table precision */
 #define GSMC_TABLES_F32 0  /* reference-identical FP32 table construction                    */
@@ -55,6 +57,11 @@ typedef struct gsmc_job {
     float* upper;
     float* alpha;
     float* beta;
+    /* GSMC_SUMMARY launches: [site_hi - site_lo] per-site reductions over the
+     * job's pairs (site_min / site_max may be NULL); per-pair outputs unused. */
+    float* site_mean;
+    float* site_min;
+    float* site_max;
 } gsmc_job;
 
 /* Message of the last failed call on this thread ("" if none). */
@@ -151,6 +158,34 @@ int gsmc_run_fb_blockwise(gsmc_context* ctx, const int32_t* pair_i, const int32_
                           int64_t core_block_sites, int64_t flank_sites, int64_t pair_batch_size,
                           int max_streams, int flags, float* mean, float* lower, float* upper, float* alpha,
                           float* beta);
+
+/* Per-site summary over n_pairs pairs, without per-pair outputs: for each
+ * site s, site_mean[s] = (sum over pairs of the posterior mean, in FP64) /
+ * n_pairs, rounded to float; site_min[s] / site_max[s] = their extremes
+ * (either may be NULL). Host arrays of n_sites floats. core_block_sites
+ * <= 0 decodes the full sequence (the reference's only mode). The sum runs
+ * in a fixed order (a butterfly over each 64-pair group, then the groups in
+ * order), so results do not depend on launch shape or pair chunking.
+ * Replaces FlowContext.run_fb_summary (src/bindings.cpp:2695-2788) and
+ * gamma_smc_flow_cached_fb_reduce_gpu (src/kernels/gamma_smc_flow.cu:1574-1683). */
+int gsmc_run_fb_summary(gsmc_context* ctx, const int32_t* pair_i, const int32_t* pair_j, int64_t n_pairs,
+                        int64_t core_block_sites, int64_t flank_sites, float* site_mean, float* site_min,
+                        float* site_max);
+
+/* Forward-only filtering of n_pairs pairs over the full sequence: the
+ * filter posterior mean at each site given the sites up to it, [n_sites]
+ * [n_pairs] host arrays (lower / upper with GSMC_CI). Same recursion as the
+ * reference's half-precision forward-only path: FP16 composed transition
+ * tables, gap = (int)(delta + 0.5) bp, explicit het emission. Replaces
+ * FlowContext.run_fwd (src/bindings.cpp:2794-2849) and
+ * gamma_smc_flow_h2_fwd_gpu (src/kernels/gamma_smc_flow.cu:1139-1265). */
+int gsmc_run_fwd(gsmc_context* ctx, const int32_t* pair_i, const int32_t* pair_j, int64_t n_pairs, int flags,
+                 float* mean, float* lower, float* upper);
+
+/* Device-resident form of gsmc_run_fwd: device pair ids and outputs,
+ * queued on `stream` without synchronising. Same reference interface. */
+int gsmc_run_fwd_device(gsmc_context* ctx, const int32_t* d_pair_i, const int32_t* d_pair_j, int64_t n_pairs,
+                        int flags, float* d_mean, float* d_lower, float* d_upper, void* stream);
 
 /* Device-resident form of gsmc_run_fb_blockwise: pair ids and outputs are
  * device pointers, outputs [n_sites][n_pairs]; the work is queued on

@@ -26,6 +26,7 @@
  * posterior identities (alpha/beta * 2Ne == mean, lower <= mean <= upper),
  * block/flank invariants and the FP64 restatement.
  */
+#define _GNU_SOURCE /* exp10f */
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -685,3 +686,111 @@ void orc_dev_bilinear(const float* tm, const float* tc, float m, float c, float*
 }
 
 int orc_seg_steps(double delta) { return seg_steps(delta); }
+
+/* ------------------------------------------------------------------ */
+/* Forward-only filter (src/kernels/gamma_smc_flow.cu:1139-1232,        */
+/* src/bindings.cpp:2794-2849)                                          */
+/* ------------------------------------------------------------------ */
+/* (int)(x) of a double: truncate, saturating; NaN -> 0. */
+static inline int d2i_rz(double x)
+{
+    if (x != x) return 0;
+    if (x >= 2147483647.0) return 2147483647;
+    if (x <= -2147483648.0) return (int)0x80000000u;
+    return (int)x;
+}
+
+/* The h2 path, FP32: tm / tc are the composed transition tables cache[k]
+ * (N layers of 2550 nodes) as the reference stores them (FP16, widened back
+ * here by the caller). Per pair and site s: for s > 0, advance by
+ * gap = (int)(pos[s] - pos[s-1] + 0.5) bp when gap > 0 (full N-bp layers
+ * first, then layer gap-1); at a het site add 1 to alpha (exact libm 10^x
+ * and log10 stand in for the reference's __exp10f / __log10f, and
+ * b_log = -2c - m is fused as nvcc fuses it); then
+ * mean = 2^(m * log2(10)) * 2Ne and, with CI, the Wilson-Hilferty bounds
+ * (the two 1.96 terms fused). Outputs site-major [S][P]. */
+int orc_fwd_only_f32(const uint8_t* G, int64_t n, int64_t S, const double* pos,
+                     const int32_t* pi, const int32_t* pj, int64_t P,
+                     const float* tm, const float* tc, int N, float Ne,
+                     float* mean, float* lower, float* upper)
+{
+    dev_consts();
+    const float two_ne = 2.0f * Ne;
+    for (int64_t p = 0; p < P; ++p)
+        if (pi[p] < 0 || pi[p] >= n || pj[p] < 0 || pj[p] >= n) return -1;
+    #pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t p = 0; p < P; ++p) {
+        const uint8_t* gi = G + (size_t)pi[p] * (size_t)S;
+        const uint8_t* gj = G + (size_t)pj[p] * (size_t)S;
+        float m = 0.0f, c = 0.0f;
+        for (int64_t s = 0; s < S; ++s) {
+            if (s > 0) {
+                int gap = d2i_rz(pos[s] - pos[s - 1] + 0.5);
+                if (gap > 0) {
+                    while (gap > N) {
+                        dev_bil2(tm + (size_t)(N - 1) * GRID, tc + (size_t)(N - 1) * GRID, m, c, &m, &c);
+                        gap -= N;
+                    }
+                    dev_bil2(tm + (size_t)(gap - 1) * GRID, tc + (size_t)(gap - 1) * GRID, m, c, &m, &c);
+                }
+            }
+            if ((gi[s] ^ gj[s]) & 1) {
+                const float alpha = exp10f(-2.0f * c) + 1.0f;
+                const float a_log = log10f(alpha);
+                const float b_log = fmaf(-2.0f, c, -m);
+                m = a_log - b_log;
+                c = -0.5f * a_log;
+            }
+            const float mg = exp2f(m * 3.321928094887362f) * two_ne;
+            const size_t idx = (size_t)s * (size_t)P + (size_t)p;
+            mean[idx] = mg;
+            if (lower && upper) {
+                const float alpha = exp10f(-2.0f * c);
+                const float inv9a = 1.0f / (9.0f * alpha);
+                const float sq = sqrtf(inv9a);
+                const float base = 1.0f - inv9a;
+                const float lo = fmaxf(fmaf(-1.96f, sq, base), 0.0f);
+                const float hi = fmaf(1.96f, sq, base);
+                lower[idx] = fmaxf(mg * lo * lo * lo, 0.0f);
+                upper[idx] = mg * hi * hi * hi;
+            }
+        }
+    }
+    return 0;
+}
+
+/* The same recursion in FP64 on FP64 tables (no FP16 rounding), exact
+ * powers and logarithms: the reference point of the FP32 path's error. */
+int orc_fwd_only_f64(const uint8_t* G, int64_t n, int64_t S, const double* pos,
+                     const int32_t* pi, const int32_t* pj, int64_t P,
+                     const double* tm, const double* tc, int N, double Ne, double* mean)
+{
+    for (int64_t p = 0; p < P; ++p)
+        if (pi[p] < 0 || pi[p] >= n || pj[p] < 0 || pj[p] >= n) return -1;
+    #pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t p = 0; p < P; ++p) {
+        const uint8_t* gi = G + (size_t)pi[p] * (size_t)S;
+        const uint8_t* gj = G + (size_t)pj[p] * (size_t)S;
+        double m = 0.0, c = 0.0;
+        for (int64_t s = 0; s < S; ++s) {
+            if (s > 0) {
+                int gap = d2i_rz(pos[s] - pos[s - 1] + 0.5);
+                if (gap > 0) {
+                    while (gap > N) {
+                        dev_bil2_d(tm + (size_t)(N - 1) * GRID, tc + (size_t)(N - 1) * GRID, m, c, &m, &c);
+                        gap -= N;
+                    }
+                    dev_bil2_d(tm + (size_t)(gap - 1) * GRID, tc + (size_t)(gap - 1) * GRID, m, c, &m, &c);
+                }
+            }
+            if ((gi[s] ^ gj[s]) & 1) {
+                const double a_log = log10(pow(10.0, -2.0 * c) + 1.0);
+                const double b_log = -2.0 * c - m;
+                m = a_log - b_log;
+                c = -0.5 * a_log;
+            }
+            mean[(size_t)s * (size_t)P + (size_t)p] = pow(10.0, m) * 2.0 * Ne;
+        }
+    }
+    return 0;
+}

@@ -75,6 +75,11 @@ def lib():
                                         ctypes.c_int64, _f64p, ctypes.c_void_p, ctypes.c_void_p]
         L.orc_forward_states_f32.argtypes = [_u8p, ctypes.c_int64, ctypes.c_int64, _f64p, ctypes.c_int32,
                                              ctypes.c_int32, _f32p, ctypes.c_int, _f32p, _f32p]
+        L.orc_fwd_only_f32.argtypes = [_u8p, ctypes.c_int64, ctypes.c_int64, _f64p, _i32p, _i32p, ctypes.c_int64,
+                                       _f32p, _f32p, ctypes.c_int, ctypes.c_float, _f32p, ctypes.c_void_p,
+                                       ctypes.c_void_p]
+        L.orc_fwd_only_f64.argtypes = [_u8p, ctypes.c_int64, ctypes.c_int64, _f64p, _i32p, _i32p, ctypes.c_int64,
+                                       _f64p, _f64p, ctypes.c_int, ctypes.c_double, _f64p]
         L.orc_fast_pow10.argtypes = [ctypes.c_float]
         L.orc_fast_pow10.restype = ctypes.c_float
         L.orc_seg_steps.argtypes = [ctypes.c_double]
@@ -212,6 +217,54 @@ def fb_window_f64(G, positions, pairs, tables64, Ne, site_start=0, block_S=None,
     return {"mean": mean, "posterior_alpha": a, "posterior_beta": b}
 
 
+def h2_tables(tables):
+    """The forward-only path's tables from the FP32 set: the composed
+    transitions cache[k] (tables 0 and 1) rounded to FP16, as the reference
+    stores them (``bindings.cpp:2098-2107``), widened back to float32."""
+    T = np.asarray(tables, np.float32)
+    return (np.ascontiguousarray(T[0].astype(np.float16).astype(np.float32)),
+            np.ascontiguousarray(T[1].astype(np.float16).astype(np.float32)))
+
+
+def fwd_only_f32(G, positions, pairs, h2, Ne, ci=False):
+    """Forward-only filter (``run_fwd``): ``mean`` and, with ``ci``, ``lower`` /
+    ``upper``, site-major float32 ``(S, P)``."""
+    G = np.ascontiguousarray(G, np.uint8)
+    pos = np.ascontiguousarray(positions, np.float64)
+    n, S = G.shape
+    pi = np.ascontiguousarray([p[0] for p in pairs], np.int32)
+    pj = np.ascontiguousarray([p[1] for p in pairs], np.int32)
+    P = len(pairs)
+    tm, tc = h2
+    out = {"mean": np.zeros((S, P), np.float32)}
+    if ci:
+        out["lower"] = np.zeros((S, P), np.float32)
+        out["upper"] = np.zeros((S, P), np.float32)
+    if P and S:
+        rc = lib().orc_fwd_only_f32(G, n, S, pos, pi, pj, P, tm.ravel(), tc.ravel(), tm.shape[0],
+                                    float(np.float32(Ne)), out["mean"].reshape(-1), _ptr(out.get("lower")),
+                                    _ptr(out.get("upper")))
+        if rc != 0:
+            raise ValueError(f"oracle fwd_only_f32 failed ({rc})")
+    return out
+
+
+def fwd_only_f64(G, positions, pairs, tables64, Ne):
+    """FP64 forward-only filter on FP64 tables (no FP16 rounding)."""
+    G = np.ascontiguousarray(G, np.uint8)
+    pos = np.ascontiguousarray(positions, np.float64)
+    n, S = G.shape
+    pi = np.ascontiguousarray([p[0] for p in pairs], np.int32)
+    pj = np.ascontiguousarray([p[1] for p in pairs], np.int32)
+    T = np.asarray(tables64, np.float64)
+    mean = np.zeros((S, len(pairs)))
+    rc = lib().orc_fwd_only_f64(G, n, S, pos, pi, pj, len(pairs), np.ascontiguousarray(T[0]).ravel(),
+                                np.ascontiguousarray(T[1]).ravel(), T.shape[1], float(Ne), mean.reshape(-1))
+    if rc != 0:
+        raise ValueError(f"oracle fwd_only_f64 failed ({rc})")
+    return {"mean": mean}
+
+
 def forward_states_f32(G, positions, pair, tables):
     G = np.ascontiguousarray(G, np.uint8)
     pos = np.ascontiguousarray(positions, np.float64)
@@ -347,6 +400,30 @@ def fb_blockwise_f64(G, positions, pairs, tables64, Ne, core, flank, exact_pow10
         r = fb_window_f64(G, positions, pairs, tables64, Ne, ps, pe - ps, exact_pow10)
         out[cs:ce] = r["mean"][cs - ps:ce - ps]
     return {"mean": out}
+
+
+def site_summary(mean):
+    """Per-site reduction of an FP32 ``(S, P)`` posterior mean over pairs
+    (the summary path's arithmetic; ``gamma_smc_flow.cu:1574-1683`` sums with
+    float atomics in no fixed order, this fixes one). Pairs go in groups of
+    64 (absent pairs 0); in each group, four parts of 16 consecutive pairs
+    are summed in order in FP64 from 0.0 and joined as
+    ``(part0 + part1) + (part2 + part3)``; the groups are added in order
+    from 0.0; mean = float(sum / P). Min and max are exact."""
+    mean = np.asarray(mean, np.float32)
+    S, P = mean.shape
+    G = -(-P // 64)
+    x = np.zeros((S, G * 64), np.float64)
+    x[:, :P] = mean
+    x = x.reshape(S, G, 4, 16)
+    t = np.zeros((S, G, 4), np.float64)
+    for i in range(16):
+        t = t + x[..., i]
+    grp = (t[..., 0] + t[..., 1]) + (t[..., 2] + t[..., 3])
+    tot = np.zeros(S, np.float64)
+    for g in range(G):
+        tot = tot + grp[:, g]
+    return {"site_mean": (tot / P).astype(np.float32), "site_min": mean.min(axis=1), "site_max": mean.max(axis=1)}
 
 
 def log_error_report(mean32, mean64):

@@ -227,3 +227,78 @@ def test_device_path_matches_host_path(core, cohort_small, flow_field_path):
         run_fb_device(ctx, d_pi, d_pj, len(pairs), 256, 64, 0, out, stream=st)
     st.synchronize()
     assert _bits_equal(out.to_host(), host["mean"])
+
+
+def test_site_summary_bit_exact_and_shape_invariant(core, cohort_small, flow_field_path):
+    """run_fb_summary (per-site mean / min / max over pairs, no per-pair
+    output) equals the oracle's fixed-order reduction of the oracle's
+    per-pair means, bit for bit, under every launch shape, full-sequence
+    and blockwise."""
+    G, pos = cohort_small
+    mu, rho = O.estimate_scaled_params(G, pos, 1.25e-8, 1e-8, 1e4)
+    T = _tables(flow_field_path, mu, rho)
+    pairs = O.all_pairs(G.shape[0]) + [(5, 5), (1, 0)]
+    ctx = core.FlowContext(G, pos, 1e4, mu, rho, flow_field_path, 0)
+    full = O.site_summary(O.fb_window_f32(G, pos, pairs, T, 1e4)["mean"])
+    blk = O.site_summary(O.fb_blockwise_f32(G, pos, pairs, T, 1e4, 200, 64)["mean"])
+    for tu in SCHEDULES:
+        _case(f"summary tuning {tu}")
+        ctx.set_tuning(**tu)
+        got = ctx.run_fb_summary(pairs)
+        assert got["n_pairs"] == len(pairs)
+        for k in ("site_mean", "site_min", "site_max"):
+            assert _bits_equal(got[k], full[k]), (tu, k)
+        got = ctx.run_fb_summary(pairs, core_block_sites=200, flank_sites=64)
+        for k in ("site_mean", "site_min", "site_max"):
+            assert _bits_equal(got[k], blk[k]), (tu, "blockwise", k)
+    _case("summary of no pairs")
+    r = ctx.run_fb_summary([])
+    assert r["site_mean"].shape == (0,) and r["n_pairs"] == 0 and "site_min" not in r
+    with pytest.raises(ValueError, match="out of range"):
+        ctx.run_fb_summary([(0, G.shape[0])])
+
+
+def test_batch_summary_jobs_match_oracle(core, cohort_small, flow_field_path):
+    """GSMC_SUMMARY batch jobs: each job's site reductions equal the
+    oracle's over that job's pairs and site range."""
+    import ctypes
+
+    from gamma_smc_cu_amd.batch import Workspace
+    from gamma_smc_cu_amd.device import DeviceArray
+
+    G, pos = cohort_small
+    n, S = G.shape
+    mu, rho = O.estimate_scaled_params(G, pos, 1.25e-8, 1e-8, 1e4)
+    T = _tables(flow_field_path, mu, rho)
+    ctx = core.FlowContext(G, pos, 1e4, mu, rho, flow_field_path, 0)
+    rng = np.random.default_rng(8)
+    specs = []
+    for lo, hi, npairs, csz, fl in ((0, S, 130, 0, 0), (50, 420, 7, 128, 30), (300, 301, 64, 0, 0)):
+        p = rng.integers(0, n, size=(npairs, 2)).astype(np.int32)
+        specs.append((lo, hi, p, csz, fl))
+    keep = []
+    jobs = (core.Job * len(specs))()
+    for k, (lo, hi, p, csz, fl) in enumerate(specs):
+        d_pi, d_pj = DeviceArray.from_host(p[:, 0].copy()), DeviceArray.from_host(p[:, 1].copy())
+        outs = [DeviceArray((hi - lo,), np.float32) for _ in range(3)]
+        keep.append((d_pi, d_pj, outs))
+        j = jobs[k]
+        j.site_lo, j.site_hi, j.core_block_sites, j.flank_sites = lo, hi, csz, fl
+        j.pair_i, j.pair_j, j.n_pairs = d_pi.ptr.value, d_pj.ptr.value, len(p)
+        j.site_mean, j.site_min, j.site_max = (o.ptr.value for o in outs)
+    _case("batch summary jobs")
+    ws = Workspace(ctx)
+    ws.launch(jobs, len(specs), core.GSMC_SUMMARY)
+    ws.wait()
+    for (lo, hi, p, csz, fl), (_, _, outs) in zip(specs, keep):
+        sub_G, sub_pos = np.ascontiguousarray(G[:, lo:hi]), pos[lo:hi]
+        pl = [tuple(map(int, x)) for x in p]
+        if csz:
+            ref = O.fb_blockwise_f32(sub_G, sub_pos, pl, T, 1e4, csz, fl)["mean"]
+        else:
+            ref = O.fb_window_f32(sub_G, sub_pos, pl, T, 1e4)["mean"]
+        want = O.site_summary(ref)
+        for o, k in zip(outs, ("site_mean", "site_min", "site_max")):
+            assert _bits_equal(o.to_host(), want[k]), (lo, hi, k)
+    with pytest.raises(ValueError, match="GSMC_SUMMARY"):
+        ws.launch(jobs, len(specs), core.GSMC_SUMMARY | core.GSMC_CI)

@@ -307,3 +307,27 @@ def test_golden_fixture_reproduced_by_oracle():
     got = golden_case.compute_with_oracle(case)
     for k, v in got.items():
         assert np.array_equal(v.view(np.uint32), case[k].view(np.uint32)), k
+
+
+def test_site_summary_restatement():
+    """The summary oracle is the plain per-site mean (to FP64 rounding of a
+    reordered sum) and the exact min / max; a dropped group or lane would
+    move the mean by far more than the tolerance."""
+    rng = np.random.default_rng(4)
+    for P in (1, 63, 64, 65, 200):
+        m = np.exp(rng.normal(8.0, 1.5, size=(37, P))).astype(np.float32)
+        r = O.site_summary(m)
+        ref = m.astype(np.float64).mean(axis=1)
+        np.testing.assert_allclose(r["site_mean"], ref.astype(np.float32), rtol=2e-7)
+        np.testing.assert_array_equal(r["site_min"], m.min(axis=1))
+        np.testing.assert_array_equal(r["site_max"], m.max(axis=1))
+    # one group by hand: four in-order parts of 16, joined pairwise
+    x = (np.arange(64, dtype=np.float64) * 1e-3 + 1.0).astype(np.float32)[None, :]
+    v = x.astype(np.float64)[0]
+    parts = []
+    for p in range(4):
+        acc = 0.0
+        for i in range(16):
+            acc = acc + v[16 * p + i]
+        parts.append(acc)
+    assert O.site_summary(x)["site_mean"][0] == np.float32(((parts[0] + parts[1]) + (parts[2] + parts[3])) / 64)
