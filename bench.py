@@ -411,6 +411,8 @@ def main():
                     help="take the launch shape from the config.tuning of a previous bench JSON line")
     ap.add_argument("--fp64-pairs", type=int, default=None,
                     help="pairs of the sample whose FP32 posterior is compared with the FP64 oracle")
+    ap.add_argument("--fwd-steps", type=int, default=2,
+                    help="also time forward-only filtering (run_fwd) this many times")
     ap.add_argument("--summary-steps", type=int, default=2,
                     help="also time the per-site summary mode (run_fb_summary) this many times")
     ap.add_argument("--no-host-stream", action="store_true",
@@ -515,6 +517,26 @@ def main():
         del src, dst
     achieved = B / (ms / 1e3) / 1e9
 
+    fwd_only = None
+    if args.fwd_steps > 0:
+        # Forward-only filtering (run_fwd) of all pairs over the full
+        # sequence into the same device output; a separate, shorter
+        # recursion (one FP16 table lookup per site, no checkpoints).
+        from gamma_smc_cu_amd.device import run_fwd_device
+
+        run_fwd_device(ctx, d_pi, d_pj, P, 0, d_out, stream=stream)
+        stream.synchronize()
+        e0, e1 = Event(), Event()
+        e0.record(stream)
+        for _ in range(args.fwd_steps):
+            run_fwd_device(ctx, d_pi, d_pj, P, 0, d_out, stream=stream)
+        e1.record(stream)
+        stream.synchronize()
+        fm = e0.elapsed_ms(e1) / args.fwd_steps
+        fwd_only = {"ms": fm, "pair_sites_per_s": pair_sites / (fm / 1e3), "steps": args.fwd_steps,
+                    "mode": "run_fwd, full sequence, mean only"}
+        log(f"forward-only: {fm:.2f} ms/call")
+
     summary = None
     if args.summary_steps > 0:
         # Fused per-site reduction over all pairs (no per-pair output): the
@@ -565,6 +587,7 @@ def main():
         "fp64_error": fp64,
         "host_stream": host,
         "site_summary": summary,
+        "forward_only": fwd_only,
         "checks": checks,
         "wall_ms_per_step": wall * 1e3 / args.steps,
         "copy_calibration_bytes": copy_bytes,

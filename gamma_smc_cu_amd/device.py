@@ -115,3 +115,15 @@ def run_fb_device(ctx, d_pi, d_pj, n_pairs, core_block_sites, flank_sites, flags
     check(lib().gsmc_run_fb_device(ctx.handle, p(d_pi), p(d_pj), int(n_pairs), int(core_block_sites),
                                    int(flank_sites), int(flags), p(d_mean), p(d_lower), p(d_upper), p(d_alpha),
                                    p(d_beta), None if stream is None else stream.handle))
+
+
+def run_fwd_device(ctx, d_pi, d_pj, n_pairs, flags, d_mean, d_lower=None, d_upper=None, stream=None):
+    """Queue a forward-only decode with device-resident pairs and outputs."""
+
+    def p(x):
+        if x is None:
+            return None
+        return x.ptr if isinstance(x, DeviceArray) else x
+
+    check(lib().gsmc_run_fwd_device(ctx.handle, p(d_pi), p(d_pj), int(n_pairs), int(flags), p(d_mean), p(d_lower),
+                                    p(d_upper), None if stream is None else stream.handle))

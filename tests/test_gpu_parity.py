@@ -302,3 +302,53 @@ def test_batch_summary_jobs_match_oracle(core, cohort_small, flow_field_path):
             assert _bits_equal(o.to_host(), want[k]), (lo, hi, k)
     with pytest.raises(ValueError, match="GSMC_SUMMARY"):
         ws.launch(jobs, len(specs), core.GSMC_SUMMARY | core.GSMC_CI)
+
+
+def _fwd_cohort():
+    """Small cohort with the forward-only path's edge cases: duplicate
+    positions (gaps that round to 0), gaps of exactly N, and gaps beyond N."""
+    from gamma_smc_cu_amd import synthetic
+
+    G, pos = synthetic.cohort(10, 500, 120_000, 19)
+    pos = pos.copy()
+    pos[100:] += 2500.0          # gap > N = 1000
+    pos[200:] += 1000.0 - (pos[200] - pos[199])  # gap exactly N
+    pos[300] = pos[299] + 0.25   # rounds to 0: no advance
+    pos[301:] += 0.25
+    return G, pos
+
+
+def test_run_fwd_matches_oracle_within_tolerance(core, flow_field_path):
+    """run_fwd against the FP32 oracle of the same recursion: the two use
+    different (correctly rounded vs 1-ulp) 10^x / log10 / 2^x, so the
+    tolerance is stated: |log mean_gpu - log mean_oracle| <= 1e-5; CI bounds
+    within 1e-4 relative (the cube in the Wilson-Hilferty bound amplifies
+    the last-ulp differences of a near-cancelling base); error against the
+    FP64 oracle on FP64 tables (no FP16 rounding) <= 3e-2 max, <= 1e-3
+    median: the reference stores this path's tables in FP16 (2^-11 relative
+    per node), which dominates."""
+    G, pos = _fwd_cohort()
+    mu, rho = O.estimate_scaled_params(G, pos, 1.25e-8, 1e-8, 1e4)
+    T = _tables(flow_field_path, mu, rho)
+    pairs = O.all_pairs(G.shape[0]) + [(4, 4)]
+    ctx = core.FlowContext(G, pos, 1e4, mu, rho, flow_field_path, 0)
+    _case("run_fwd with CI")
+    got = ctx.run_fwd(pairs, mean_only=False)
+    ref = O.fwd_only_f32(G, pos, pairs, O.h2_tables(T), 1e4, ci=True)
+    err = np.abs(np.log(got["mean"]) - np.log(ref["mean"]))
+    print(f"[fwd] max |dlog mean| vs fp32 oracle {err.max():.3e}", flush=True)
+    assert err.max() <= 1e-5
+    for k in ("lower", "upper"):
+        np.testing.assert_allclose(got[k], ref[k], rtol=1e-4, atol=1e-6 * float(ref["mean"].max()))
+    assert np.all(got["lower"] <= got["mean"]) and np.all(got["mean"] <= got["upper"])
+    r64 = O.fwd_only_f64(G, pos, pairs, O.tables_f64(flow_field_path, 1000, 1e4, mu, rho), 1e4)
+    e64 = np.abs(np.log(got["mean"]) - np.log(r64["mean"]))
+    print(f"[fwd] vs fp64 oracle: max {e64.max():.3e} median {np.median(e64):.3e}", flush=True)
+    assert e64.max() <= 3e-2 and np.median(e64) <= 1e-3
+    _case("run_fwd mean only, chunked pair order")
+    m_only = ctx.run_fwd(pairs[::-1], mean_only=True)
+    assert set(m_only) == {"mean"}
+    assert _bits_equal(m_only["mean"][:, ::-1], got["mean"])
+    assert ctx.run_fwd([], mean_only=True)["mean"].shape == (G.shape[1], 0)
+    with pytest.raises(ValueError, match="out of range"):
+        ctx.run_fwd([(0, 99)])

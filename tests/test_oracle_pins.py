@@ -331,3 +331,36 @@ def test_site_summary_restatement():
             acc = acc + v[16 * p + i]
         parts.append(acc)
     assert O.site_summary(x)["site_mean"][0] == np.float32(((parts[0] + parts[1]) + (parts[2] + parts[3])) / 64)
+
+
+def test_forward_only_oracle_against_independent_walk():
+    """The forward-only oracle (run_fwd's recursion) against a float64
+    walk with scipy interpolation on the same FP16-rounded tables: gap
+    (int)(delta + 0.5) with nothing at site 0 or for gaps rounding to 0,
+    full N-bp layers first, explicit het emission, mean = 10^m * 2Ne."""
+    Ne, mu, rho, N = 1e4, 1.25e-8, 1e-8, 1000
+    T = O.tables_f32(FF, N, Ne, mu, rho)
+    tm, tc = O.h2_tables(T)
+    # the FP16 tables are the reference's cache[k] rounded to nearest even
+    assert np.array_equal(tm, T[0].astype(np.float16).astype(np.float32))
+    G = np.array([[0, 1, 0, 1, 1, 0, 1], [1, 1, 0, 0, 1, 1, 0], [0, 0, 0, 1, 0, 1, 1]], np.uint8)
+    pos = np.array([499.0, 3499.0, 3499.2, 3620.5, 5001.5, 6001.5, 6002.0])
+    layers = np.stack([tm, tc], axis=-1)  # (N, 2550, 2)
+    for pair in [(1, 0), (2, 0), (2, 1), (1, 1)]:
+        het = (G[pair[0]] ^ G[pair[1]]) & 1
+        m = c = 0.0
+        want = []
+        for s in range(G.shape[1]):
+            if s > 0:
+                gap = int(pos[s] - pos[s - 1] + 0.5)
+                if gap > 0:
+                    while gap > N:
+                        m, c = _interp(layers[N - 1], m, c)
+                        gap -= N
+                    m, c = _interp(layers[gap - 1], m, c)
+            if het[s]:
+                a_log = np.log10(10.0 ** (-2.0 * c) + 1.0)
+                m, c = a_log - (-2.0 * c - m), -0.5 * a_log
+            want.append(10.0 ** m * 2.0 * Ne)
+        got = O.fwd_only_f32(G, pos, [pair], (tm, tc), Ne)["mean"][:, 0]
+        np.testing.assert_allclose(got, want, rtol=2e-4)
