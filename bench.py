@@ -411,6 +411,8 @@ def main():
                     help="take the launch shape from the config.tuning of a previous bench JSON line")
     ap.add_argument("--fp64-pairs", type=int, default=None,
                     help="pairs of the sample whose FP32 posterior is compared with the FP64 oracle")
+    ap.add_argument("--codes-steps", type=int, default=2,
+                    help="also time the decode with uint8 log-quantized output this many times")
     ap.add_argument("--fwd-steps", type=int, default=2,
                     help="also time forward-only filtering (run_fwd) this many times")
     ap.add_argument("--summary-steps", type=int, default=2,
@@ -537,6 +539,27 @@ def main():
                     "mode": "run_fwd, full sequence, mean only"}
         log(f"forward-only: {fm:.2f} ms/call")
 
+    codes = None
+    if args.codes_steps > 0:
+        # The same decode with the mean written as uint8 log codes (1 B per
+        # pair-site instead of 4).
+        from gamma_smc_cu_amd._core import GSMC_Q8
+
+        d_q = DeviceArray((S, P), np.uint8)
+        run_fb_device(ctx, d_pi, d_pj, P, core, flank, GSMC_Q8, d_q, stream=stream)
+        stream.synchronize()
+        e0, e1 = Event(), Event()
+        e0.record(stream)
+        for _ in range(args.codes_steps):
+            run_fb_device(ctx, d_pi, d_pj, P, core, flank, GSMC_Q8, d_q, stream=stream)
+        e1.record(stream)
+        stream.synchronize()
+        cm = e0.elapsed_ms(e1) / args.codes_steps
+        codes = {"ms": cm, "pair_sites_per_s": pair_sites / (cm / 1e3), "steps": args.codes_steps,
+                 "output_bytes": P * S, "mode": "uint8 log-quantized mean codes"}
+        log(f"uint8 codes: {cm:.2f} ms/step")
+        del d_q
+
     summary = None
     if args.summary_steps > 0:
         # Fused per-site reduction over all pairs (no per-pair output): the
@@ -588,6 +611,7 @@ def main():
         "host_stream": host,
         "site_summary": summary,
         "forward_only": fwd_only,
+        "codes_u8": codes,
         "checks": checks,
         "wall_ms_per_step": wall * 1e3 / args.steps,
         "copy_calibration_bytes": copy_bytes,

@@ -28,6 +28,8 @@ LIB_PATH = os.path.join(_HERE, "libgsmc.so")
 GSMC_CI = 1
 GSMC_POSTERIOR = 2
 GSMC_SUMMARY = 4
+GSMC_Q8 = 8
+GSMC_Q4 = 16
 _E_INVAL, _E_HIP, _E_NOMEM, _E_IO = -1, -2, -3, -4
 
 # Every symbol include/gamma_smc_amd.h declares.
@@ -229,6 +231,30 @@ def _split_pairs(pairs):
 _PRECISION = {"fp32": 0, "fp64": 1}
 
 
+def check_pairs_host(pi, pj, n_haps):
+    if pi.size and (pi.min() < 0 or pj.min() < 0 or pi.max() >= n_haps or pj.max() >= n_haps):
+        raise ValueError("pair index out of range")
+
+
+def dequantize(codes, bits, Ne):
+    """Posterior means represented by log-quantized codes (the inverse of
+    the code map, at the centre of each code's bin edge):
+    ``exp(code / levels * log(20 Ne))``; uint4 codes are unpacked to one
+    row per site first (``n_sites`` rows are returned for an even count;
+    the caller drops the padding row of an odd count)."""
+    codes = np.asarray(codes, np.uint8)
+    if bits == 4:
+        lo, hi = codes & 0x0F, codes >> 4
+        codes = np.stack([lo, hi], axis=1).reshape(-1, codes.shape[1])
+        levels = 15.0
+    elif bits == 8:
+        levels = 255.0
+    else:
+        raise ValueError("bits must be 4 or 8")
+    log_max = float(np.log(np.float32(20.0) * np.float32(Ne)))
+    return np.exp(codes.astype(np.float64) / levels * log_max).astype(np.float32)
+
+
 class FlowContext:
     """Device context for flow-field Gamma-SMC decoding.
 
@@ -305,6 +331,33 @@ class FlowContext:
         sched = {"static": 0, "persistent": 1}[schedule]
         check(lib().gsmc_context_set_tuning(self._h, int(checkpoint_k), int(pairs_per_lane), sched,
                                             int(bool(xcd_order)), int(min_waves_per_launch)))
+
+    def run_fb_codes(self, pairs, bits=8, core_block_sites=0, flank_sites=0):
+        """Blockwise decode whose posterior mean comes back as log-quantized
+        codes (``gamma_smc.cu:23-39``, ``bindings.cpp:1680-1810``): uint8
+        ``(n_sites, n_pairs)`` for ``bits=8``; for ``bits=4`` two sites per
+        byte, ``(ceil(n_sites / 2), n_pairs)``, even site in the low nibble.
+        Codes map ``log(max(mean, 1))`` linearly from ``[log_min, log_max]``
+        = ``[0, log(20 Ne)]`` onto 0..255 (0..15); :func:`dequantize`
+        inverts them. ``core_block_sites <= 0`` decodes the full sequence."""
+        if bits not in (4, 8):
+            raise ValueError("bits must be 4 or 8")
+        from .device import DeviceArray, run_fb_device
+
+        pi, pj = _split_pairs(pairs)
+        P, S = pi.shape[0], self._S
+        rows = S if bits == 8 else (S + 1) // 2
+        core = int(core_block_sites) if core_block_sites > 0 else max(S, 1)
+        flank = int(flank_sites) if core_block_sites > 0 else 0
+        codes = np.zeros((rows, P), np.uint8)
+        if P and S:
+            check_pairs_host(pi, pj, self._n)
+            d_pi, d_pj = DeviceArray.from_host(pi), DeviceArray.from_host(pj)
+            d_q = DeviceArray((rows, P), np.uint8)
+            run_fb_device(self, d_pi, d_pj, P, core, flank, GSMC_Q8 if bits == 8 else GSMC_Q4, d_q)
+            d_q.to_host(codes)
+        return {"codes": codes, "bits": bits, "log_min": 0.0, "log_max": float(np.log(np.float32(20.0) * self._Ne)),
+                "blocks": block_windows(S, core, flank)}
 
     def run_fwd(self, pairs, mean_only=True):
         """Forward-only filtering over the full sequence

@@ -121,6 +121,8 @@ struct FbArgs {
     DevTables t;
     float unscale;             // 2 Ne
     const float2* prior;       // prior[q] = q full N-bp hom stretches applied to the prior (0, 0)
+    int32_t qbits;             // mean output encoding: 0 float, 8 / 4 log-quantized codes
+    float q_inv_range;         // 1 / log(20 Ne): codes span [log 1, log 20 Ne]
     double* psum;              // summary launches: per 64-pair group and site, the sum of the means
     float* pmin;               // ... their minimum
     float* pmax;               // ... their maximum
@@ -357,6 +359,50 @@ __device__ __forceinline__ void summarize(const FbArgs& a, const JobDev& j, cons
     }
 }
 
+// Log-quantized code of a posterior mean (gamma_smc.cu:23-39): the natural
+// log of max(mean, 1 generation) mapped linearly from [0, log 20Ne] onto
+// 0 .. levels and rounded half up (the scale-and-add is fused as nvcc
+// fuses it), then clamped.
+__device__ __forceinline__ uint32_t log_code(float mean, float inv_range, float levels)
+{
+    const float x = logf(fmaxf(mean, 1.0f)) * inv_range;
+    const float q = fmaf(x, levels, 0.5f);
+    return (uint32_t)fminf(fmaxf(q, 0.0f), levels);
+}
+
+// Compact output of one window: uint8 codes [rows][P], or uint4 codes two
+// sites per byte ([ceil(rows / 2)][P], even row in the low nibble). Windows
+// start at an even row (block cores have an even length), so no byte is
+// shared between windows; an odd last row leaves the high nibble 0.
+template <int K, int NP, bool FAST>
+__device__ __forceinline__ void store_codes(const FbArgs& a, const JobDev& j, const Lanes<NP>& L, int64_t ws,
+                                            int64_t cnt, const float (&out)[NP][K])
+{
+    uint8_t* base = reinterpret_cast<uint8_t*>(j.mean);
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+        if (!L.live[q]) continue;
+        const size_t col = L.col[q] >> 2;
+        if (a.qbits == 8) {
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                if (FAST || k < cnt)
+                    __builtin_nontemporal_store((uint8_t)log_code(out[q][k], a.q_inv_range, 255.0f),
+                                                base + (size_t)(ws + k - j.row0) * (size_t)j.P + col);
+        } else {
+#pragma unroll
+            for (int k = 0; k < K; k += 2) {
+                if (FAST || k < cnt) {
+                    uint32_t v = log_code(out[q][k], a.q_inv_range, 15.0f);
+                    if (FAST || k + 1 < cnt) v |= log_code(out[q][k + 1], a.q_inv_range, 15.0f) << 4;
+                    __builtin_nontemporal_store((uint8_t)v,
+                                                base + (size_t)((ws + k - j.row0) >> 1) * (size_t)j.P + col);
+                }
+            }
+        }
+    }
+}
+
 // Phase-3 step over one checkpoint window, software-pipelined: the
 // backward walk of window w (cnt sites at ws, forward states in F) runs in
 // the same loop as the forward recompute of window w - 1 (K sites) when
@@ -411,6 +457,10 @@ __device__ __forceinline__ void window_step(const FbArgs& a, const JobDev& j, co
     }
     if (SUM) {
         summarize<K, NP>(a, j, L, ws, cnt, out, red);
+        return;
+    }
+    if (!CI && !POST && a.qbits != 0) {
+        store_codes<K, NP, FAST>(a, j, L, ws, cnt, out);
         return;
     }
 #pragma unroll

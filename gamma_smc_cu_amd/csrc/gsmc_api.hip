@@ -639,6 +639,8 @@ void launch_plan(gsmc_batch* b, const std::vector<JobSpec>& jobs, int flags, hip
     a.W = ctx->W;
     a.desc = ctx->desc.as<uint32_t>();
     a.prior = ctx->prior.as<float2>();
+    a.qbits = (flags & GSMC_Q8) ? 8 : (flags & GSMC_Q4) ? 4 : 0;
+    a.q_inv_range = 1.0f / logf(20.0f * ctx->Ne);  // codes span [log 1, log 20Ne] (bindings.cpp:1698-1700)
     a.jobs = reinterpret_cast<const JobDev*>(b->meta.p);
     a.blocks = reinterpret_cast<const BlockDev*>(b->meta.as<char>() + pl.off_blocks);
     a.items = reinterpret_cast<const ItemDev*>(b->meta.as<char>() + pl.off_items);
@@ -681,6 +683,21 @@ void launch_plan(gsmc_batch* b, const std::vector<JobSpec>& jobs, int flags, hip
 
 int n_outputs(int flags) { return 1 + ((flags & GSMC_CI) ? 2 : 0) + ((flags & GSMC_POSTERIOR) ? 2 : 0); }
 
+// Output encodings: log-quantized codes replace the float mean only, and
+// uint4 codes pack two rows per byte, so every block core must start at an
+// even row of its job.
+void check_codes(int flags, const std::vector<Block>& blocks, int64_t row0)
+{
+    const bool codes = (flags & (GSMC_Q8 | GSMC_Q4)) != 0;
+    if (!codes) return;
+    require(!((flags & GSMC_Q8) && (flags & GSMC_Q4)), "GSMC_Q8 and GSMC_Q4 are exclusive");
+    require(!(flags & (GSMC_CI | GSMC_POSTERIOR | GSMC_SUMMARY)),
+            "quantized output cannot be combined with GSMC_CI, GSMC_POSTERIOR or GSMC_SUMMARY");
+    if (flags & GSMC_Q4)
+        for (const Block& b : blocks)
+            require(((int64_t)b.cs - row0) % 2 == 0, "uint4 output needs an even core_block_sites");
+}
+
 void check_pairs(const gsmc_context* ctx, const int32_t* pi, const int32_t* pj, int64_t P)
 {
     for (int64_t p = 0; p < P; ++p)
@@ -718,6 +735,8 @@ void scatter_rows(float* dst, int64_t ld_dst, int64_t col0, const float* src, in
 void run_to_host(gsmc_context* ctx, const int32_t* pi, const int32_t* pj, int64_t P, const std::vector<Block>& blocks,
                  int64_t pair_batch, int flags, float* const* host_out)
 {
+    require(!(flags & (GSMC_Q8 | GSMC_Q4 | GSMC_SUMMARY)),
+            "host-output runs write float rows; use gsmc_run_fb_device, a batch launch or gsmc_run_fb_summary");
     init_streams(ctx);
     for (auto& hb : ctx->host_batch)
         if (!hb) hb = new_batch(ctx);
@@ -1104,11 +1123,13 @@ int gsmc_run_fb_device(gsmc_context* ctx, const int32_t* d_pi, const int32_t* d_
         require(d_mean != nullptr, "mean is NULL");
         require(!(flags & GSMC_CI) || (d_lower && d_upper), "lower/upper required with GSMC_CI");
         require(!(flags & GSMC_POSTERIOR) || (d_alpha && d_beta), "alpha/beta required with GSMC_POSTERIOR");
+        require(!(flags & GSMC_SUMMARY), "use gsmc_run_fb_summary for per-site summaries");
         std::lock_guard<std::mutex> lk(ctx->mu);
         HIP_OK(hipSetDevice(ctx->device));
         if (!ctx->dev_batch) ctx->dev_batch = new_batch(ctx);
         JobSpec js;
         js.blocks = make_blocks(ctx->S, core, flank);
+        check_codes(flags, js.blocks, 0);
         js.pi = d_pi;
         js.pj = d_pj;
         js.P = P;
@@ -1336,6 +1357,7 @@ int gsmc_batch_launch(gsmc_batch* b, const gsmc_job* jobs, int64_t n_jobs, int f
             js.P = j.n_pairs;
             js.out = Outputs{j.mean, j.lower, j.upper, j.alpha, j.beta, j.site_mean, j.site_min, j.site_max};
             js.row0 = j.site_lo;
+            check_codes(flags, js.blocks, js.row0);
             specs.push_back(std::move(js));
         }
         std::lock_guard<std::mutex> lk(ctx->mu);

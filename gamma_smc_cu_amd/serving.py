@@ -104,6 +104,7 @@ class Query:
     site_hi: int
     core_block_sites: int = 32_768
     flank_sites: int = 2048
+    bits: int = 0              # output: 0 float32 means; 8 / 4 log-quantized codes
     qid: int = -1
     arrival: float = math.nan
     launched: float = math.nan
@@ -129,6 +130,15 @@ class Query:
     @property
     def latency(self):
         return self.finished - self.arrival
+
+    @property
+    def out_shape(self):
+        rows = self.n_sites if self.bits != 4 else (self.n_sites + 1) // 2
+        return rows, self.n_pairs
+
+    @property
+    def out_dtype(self):
+        return np.float32 if self.bits == 0 else np.uint8
 
 
 
@@ -234,8 +244,13 @@ class Server:
             raise ValueError("haplotypes must be at least two distinct ids in [0, n_haps)")
         if q.flank_sites < 0:
             raise ValueError("flank_sites must be non-negative.")
+        if q.bits not in (0, 8, 4):
+            raise ValueError("bits must be 0 (float means), 8 or 4")
+        if q.bits == 4 and q.core_block_sites > 0 and q.n_sites > q.core_block_sites and q.core_block_sites % 2:
+            raise ValueError("uint4 output needs an even core_block_sites")
         q.pairs = subset_pairs(h)
-        q._bytes = 4 * q.pair_sites
+        rows, cols = q.out_shape
+        q._bytes = rows * cols * np.dtype(q.out_dtype).itemsize
         q._ck = self.ctx.checkpoint_bytes(q.n_sites, q.core_block_sites, q.flank_sites, q.n_pairs)
         q._items = -(-q.n_pairs // 64) * _n_blocks(q.n_sites, q.core_block_sites, q.flank_sites)
         if _align(q._bytes) > self.arena.size or q._ck > self.ck_cap or q._items > self.item_cap:
@@ -287,10 +302,12 @@ class Server:
         return out
 
     def result(self, q):
-        """Copy a finished query's output to the host: ``(n_sites, n_pairs)``."""
+        """Copy a finished query's output to the host: float32 means
+        ``(n_sites, n_pairs)``, or uint8 codes (``bits`` 8: same shape; 4:
+        ``(ceil(n_sites / 2), n_pairs)``, see ``FlowContext.run_fb_codes``)."""
         if q.qid not in self.completed:
             raise ValueError("query has not finished")
-        out = np.empty((q.n_sites, q.n_pairs), np.float32)
+        out = np.empty(q.out_shape, q.out_dtype)
         src = self.d_arena.offset(q._off)
         _core.check(_core.lib().gsmc_memcpy_dtoh(ctypes.c_void_p(out.ctypes.data), src, out.nbytes))
         return out
@@ -306,7 +323,7 @@ class Server:
         batch, ck, items = [], 0, 0
         while self.pending and len(batch) < self.query_cap:
             q = self.pending[0]
-            if batch and (ck + q._ck > self.ck_cap or items + q._items > self.item_cap):
+            if batch and (ck + q._ck > self.ck_cap or items + q._items > self.item_cap or q.bits != batch[0].bits):
                 break
             off = self.arena.alloc(q._bytes)
             if off is None:
@@ -338,8 +355,9 @@ class Server:
             at += P
         _core.check(_core.lib().gsmc_memcpy_htod_async(slot.d_pairs.ptr, slot.stage.ptr, 8 * total,
                                                        slot.stream.handle))
+        flags = {0: 0, 8: _core.GSMC_Q8, 4: _core.GSMC_Q4}[batch[0].bits]
         slot.start.record(slot.stream)
-        slot.ws.launch(jobs, len(batch), 0, slot.stream)
+        slot.ws.launch(jobs, len(batch), flags, slot.stream)
         slot.done.record(slot.stream)
         now = self.clock()
         for q in batch:

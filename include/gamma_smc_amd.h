@@ -31,6 +31,11 @@ extern "C" {
 #define GSMC_POSTERIOR 2   /* also write posterior alpha / beta (return_posterior=True)       */
 #define GSMC_SUMMARY 4     /* batch jobs: write per-site mean / min / max over the pairs
                               (site_mean, site_min, site_max) instead of per-pair rows       */
+#define GSMC_Q8 8          /* mean receives uint8 log-quantized codes [rows][P] instead of
+                              floats: code = clamp(floor(ln(max(mean, 1)) / ln(20 Ne) * 255
+                              + 0.5), 0, 255) (gamma_smc.cu:23-39, bindings.cpp:1680-1810)  */
+#define GSMC_Q4 16         /* uint4 codes (levels 15), two rows per byte, even row in the
+                              low nibble: [ceil(rows / 2)][P]; block cores must be even     */
 
 /* table precision */
 #define GSMC_TABLES_F32 0  /* reference-identical FP32 table construction                    */

@@ -794,3 +794,17 @@ int orc_fwd_only_f64(const uint8_t* G, int64_t n, int64_t S, const double* pos,
     }
     return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* Log-quantized output codes (src/kernels/gamma_smc.cu:23-39)          */
+/* ------------------------------------------------------------------ */
+/* code = (unsigned)clamp(fma(logf(max(mean, 1)) * inv_range, levels, 0.5),
+ * 0, levels) for n means; levels 255 or 15. */
+void orc_log_codes(const float* mean, int64_t n, float inv_range, float levels, uint8_t* out)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        const float x = logf(fmaxf(mean[i], 1.0f)) * inv_range;
+        const float q = fmaf(x, levels, 0.5f);
+        out[i] = (uint8_t)fminf(fmaxf(q, 0.0f), levels);
+    }
+}

@@ -80,6 +80,8 @@ def lib():
                                        ctypes.c_void_p]
         L.orc_fwd_only_f64.argtypes = [_u8p, ctypes.c_int64, ctypes.c_int64, _f64p, _i32p, _i32p, ctypes.c_int64,
                                        _f64p, _f64p, ctypes.c_int, ctypes.c_double, _f64p]
+        L.orc_log_codes.argtypes = [_f32p, ctypes.c_int64, ctypes.c_float, ctypes.c_float, _u8p]
+        L.orc_log_codes.restype = None
         L.orc_fast_pow10.argtypes = [ctypes.c_float]
         L.orc_fast_pow10.restype = ctypes.c_float
         L.orc_seg_steps.argtypes = [ctypes.c_double]
@@ -263,6 +265,24 @@ def fwd_only_f64(G, positions, pairs, tables64, Ne):
     if rc != 0:
         raise ValueError(f"oracle fwd_only_f64 failed ({rc})")
     return {"mean": mean}
+
+
+def log_codes(mean, bits, Ne):
+    """Log-quantized codes of FP32 means ``(S, P)`` (``gamma_smc.cu:23-39``):
+    uint8 ``(S, P)``, or for 4 bits ``(ceil(S / 2), P)`` with the even site
+    in the low nibble (``gamma_smc.cu:185-205``)."""
+    mean = np.ascontiguousarray(mean, np.float32)
+    inv = np.float32(1.0) / np.float32(np.log(np.float32(20.0) * np.float32(Ne)))
+    levels = 255.0 if bits == 8 else 15.0
+    q = np.zeros(mean.shape, np.uint8)
+    if mean.size:
+        lib().orc_log_codes(mean.reshape(-1), mean.size, float(inv), levels, q.reshape(-1))
+    if bits == 8:
+        return q
+    S, P = q.shape
+    if S % 2:
+        q = np.concatenate([q, np.zeros((1, P), np.uint8)], axis=0)
+    return (q[0::2] | (q[1::2] << 4)).astype(np.uint8)
 
 
 def forward_states_f32(G, positions, pair, tables):

@@ -352,3 +352,43 @@ def test_run_fwd_matches_oracle_within_tolerance(core, flow_field_path):
     assert ctx.run_fwd([], mean_only=True)["mean"].shape == (G.shape[1], 0)
     with pytest.raises(ValueError, match="out of range"):
         ctx.run_fwd([(0, 99)])
+
+
+def _nibbles(q):
+    return np.stack([q & 0x0F, q >> 4], axis=1).reshape(-1, q.shape[1])
+
+
+@pytest.mark.parametrize("bits", [8, 4])
+def test_log_codes_match_oracle(core, cohort_small, flow_field_path, bits):
+    """Log-quantized mean codes equal the oracle's codes of the oracle's
+    FP32 means. The GPU's log and the oracle's libm log may differ in the
+    last ulp, which can move a value sitting on a code boundary by one
+    code: at most one code, in at most 1e-3 of the entries."""
+    from gamma_smc_cu_amd._core import dequantize
+
+    G, pos = cohort_small
+    mu, rho = O.estimate_scaled_params(G, pos, 1.25e-8, 1e-8, 1e4)
+    T = _tables(flow_field_path, mu, rho)
+    pairs = O.all_pairs(G.shape[0]) + [(5, 5)]
+    for S_used, csz, fl in ((G.shape[1], 200, 64), (G.shape[1] - 1, 0, 0)):
+        _case(f"codes bits={bits} S={S_used} core={csz}")
+        Gs, ps_ = np.ascontiguousarray(G[:, :S_used]), pos[:S_used]
+        ctx = core.FlowContext(Gs, ps_, 1e4, mu, rho, flow_field_path, 0)
+        if csz:
+            ref = O.fb_blockwise_f32(Gs, ps_, pairs, T, 1e4, csz, fl)["mean"]
+        else:
+            ref = O.fb_window_f32(Gs, ps_, pairs, T, 1e4)["mean"]
+        want = O.log_codes(ref, bits, 1e4)
+        got = ctx.run_fb_codes(pairs, bits=bits, core_block_sites=csz, flank_sites=fl)
+        assert got["codes"].shape == want.shape and got["bits"] == bits
+        a, b = (got["codes"], want) if bits == 8 else (_nibbles(got["codes"]), _nibbles(want))
+        d = np.abs(a.astype(np.int32) - b.astype(np.int32))
+        assert d.max() <= 1 and np.count_nonzero(d) <= 1e-3 * d.size, (d.max(), np.count_nonzero(d))
+        # decoded values sit within half a code of the mean (for means in range)
+        deq = dequantize(got["codes"], bits, 1e4)[:S_used]
+        half_bin = got["log_max"] / (255.0 if bits == 8 else 15.0)
+        inr = (ref >= 1.0) & (ref <= 2e5)
+        assert np.all(np.abs(np.log(deq[inr]) - np.log(ref[inr])) <= half_bin * 0.5 + half_bin + 1e-6)
+    if bits == 4:
+        with pytest.raises(ValueError, match="even core_block_sites"):
+            ctx.run_fb_codes(pairs, bits=4, core_block_sites=201, flank_sites=10)

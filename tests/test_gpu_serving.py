@@ -127,3 +127,38 @@ def test_rejects_queries_that_cannot_fit(resident):
     with pytest.raises(ValueError, match="distinct"):
         srv.submit(serving.Query([3, 3], 0, 10))
     assert srv.idle()
+
+
+def test_code_queries_match_float_queries(resident):
+    """Queries asking for uint8 / uint4 codes batch only with queries of
+    the same encoding, and their codes are those of the float queries'
+    means (to one code at a log ulp boundary)."""
+    ctx = resident[0]
+    base = _queries(3, 12)
+    mixed = []
+    for k, q in enumerate(base):
+        mixed.append(serving.Query(q.haplotypes, q.site_lo, q.site_hi, 400, 60, bits=(0, 8, 4)[k % 3]))
+    _case("serving mixed encodings")
+    srv = serving.Server(ctx, arena_bytes=64 << 20, n_slots=2, ck_bytes_per_launch=16 << 20, max_items_per_launch=4096)
+    got = {}
+
+    def keep(q):
+        got[q.qid] = srv.result(q)
+        srv.release(q)
+
+    serving.run_open_loop(srv, mixed, np.zeros(len(mixed)), keep)
+    floats = _alone(ctx, [q for q in base])
+    for q, f in zip(mixed, floats):
+        r = got[q.qid]
+        if q.bits == 0:
+            assert _bits_equal(r, f)
+            continue
+        want = O.log_codes(f, q.bits, 1e4)
+        assert r.shape == want.shape and r.dtype == np.uint8
+        a, b = (r, want) if q.bits == 8 else (_nib(r), _nib(want))
+        d = np.abs(a.astype(np.int32) - b.astype(np.int32))
+        assert d.max() <= 1 and np.count_nonzero(d) <= max(1, 1e-3 * d.size)
+
+
+def _nib(q):
+    return np.stack([q & 0x0F, q >> 4], axis=1).reshape(-1, q.shape[1])
