@@ -103,8 +103,77 @@ def _density_change(t: np.ndarray, alpha: float, beta: float) -> np.ndarray:
     return (from_below - from_above) + (0.5 * e2 - 0.5 - t) * pdf + (1.0 - e2) * sf
 
 
+def _epochs(Ne_values, epoch_boundaries, Ne_ref):
+    """Piecewise-constant coalescence rate in scaled time s = T / (2 Ne_ref)
+    (``demographic.py:149-158``): epoch k starts at boundary k (generations,
+    the first at 0; the last epoch is open-ended), rate lam_k = Ne_ref / N_k,
+    and the cumulative rate Lam(s) is L_k + lam_k (s - t_k) inside epoch k."""
+    M = len(Ne_values)
+    t_b = np.zeros(M + 1)
+    t_b[1:M] = np.asarray(epoch_boundaries, np.float64)[1:M] / (2.0 * Ne_ref)
+    t_ext = np.append(t_b[:M], 1e15)
+    lam = float(Ne_ref) / np.asarray(Ne_values, np.float64)
+    L = np.zeros(M + 1)
+    for k in range(1, M):
+        L[k] = L[k - 1] + lam[k - 1] * (t_b[k] - t_b[k - 1])
+    return lam, t_ext, L
+
+
+def _density_change_demographic(t, alpha, beta, lam, t_ext, L):
+    """Change of the TMRCA density after one recombination step when the
+    coalescence rate is piecewise constant (``demographic.py:47-130``):
+
+        e^{-Lam(t)} (I+(t) - I-(t)) + (e^{-2 Lam}/2 - 1/2 - Lam) pdf + (1 - e^{-2 Lam}) sf
+
+    with I+-(t) = int_0^t e^{+-Lam(s)} Gamma_pdf(s) ds summed epoch by epoch in
+    closed form: inside epoch k the integrand is e^{+-(L_k - lam_k t_k)} times
+    a Gamma density of rate c = beta -+ lam_k, so each piece is
+    e^{...} (beta / c)^alpha [P(alpha, c b) - P(alpha, c a)]. As upstream:
+    pieces whose scale is below e^-500 are dropped, c ~ 0 uses the power
+    form, and pieces with c < 0 (a rate above beta) are left out."""
+    from scipy.special import gammainc, gammaincc, gammaln
+
+    t = np.maximum(np.asarray(t, dtype=np.float64), 1e-15)
+    M = len(lam)
+    k_of_t = np.clip(np.searchsorted(t_ext[:M], t, side="right") - 1, 0, M - 1)
+    Lam = L[k_of_t] + lam[k_of_t] * (t - t_ext[k_of_t])
+    ip = np.zeros_like(t)
+    im = np.zeros_like(t)
+    for k in range(M):
+        a_k = t_ext[k]
+        b_k = np.minimum(t, t_ext[k + 1] if k < M - 1 else 1e15)
+        on = t > a_k + 1e-15
+        if not on.any():
+            break
+        shift = L[k] - lam[k] * t_ext[k]
+        c = beta - lam[k]
+        if c > 1e-10:
+            log_scale = shift + alpha * (np.log(beta) - np.log(c))
+            if log_scale > -500.0:
+                lo = gammainc(alpha, c * max(a_k, 1e-30)) if a_k > 1e-15 else 0.0
+                hi = np.where(on, gammainc(alpha, c * np.maximum(b_k, 1e-30)), 0.0)
+                ip += np.where(on, np.exp(log_scale) * (hi - lo), 0.0)
+        elif abs(c) <= 1e-10 and alpha > 0:
+            coef = np.exp(shift + alpha * np.log(beta) - gammaln(alpha))
+            hi = np.where(on, np.power(np.maximum(b_k, 1e-30), alpha), 0.0)
+            lo = max(a_k, 1e-30) ** alpha if a_k > 1e-15 else 0.0
+            ip += np.where(on, coef * (hi - lo) / alpha, 0.0)
+        c = beta + lam[k]
+        log_scale = -shift + alpha * (np.log(beta) - np.log(c))
+        if log_scale > -500.0:
+            lo = gammainc(alpha, c * max(a_k, 1e-30)) if a_k > 1e-15 else 0.0
+            hi = np.where(on, gammainc(alpha, c * np.maximum(b_k, 1e-30)), 0.0)
+            im += np.where(on, np.exp(log_scale) * (hi - lo), 0.0)
+    log_pdf = alpha * np.log(beta) - gammaln(alpha) + (alpha - 1.0) * np.log(t) - beta * t
+    pdf = np.exp(np.where(log_pdf > -700.0, log_pdf, -700.0))
+    sf = gammaincc(alpha, beta * t)
+    e2 = np.exp(-2.0 * Lam)
+    return np.exp(-Lam) * (ip - im) + (0.5 * e2 - 0.5 - Lam) * pdf + (1.0 - e2) * sf
+
+
 def generate_flow_field(mean_min=1e-5, mean_max=1e2, cv_min=1e-2, cv_max=1.0,
-                        mean_n=MEAN_N, cv_n=CV_N, n_steps=500):
+                        mean_n=MEAN_N, cv_n=CV_N, n_steps=500, Ne_values=None, epoch_boundaries=None,
+                        Ne_ref=10000.0):
     """Return the (u, v) field, each ``(mean_n, cv_n)`` float64, mean-major.
 
     Grid nodes are log-spaced: ``log10(mean)`` over [log10 mean_min,
@@ -112,6 +181,12 @@ def generate_flow_field(mean_min=1e-5, mean_max=1e2, cv_min=1e-2, cv_max=1.0,
     (``demographic.py:161-162``). Nodes with a degenerate Gamma (alpha or
     beta below 1e-8, alpha above 1e8) or fewer than three usable time points
     keep a zero flow, as upstream does (``demographic.py:174-195``).
+
+    With ``Ne_values`` (one population size per epoch) and
+    ``epoch_boundaries`` (epoch start times in generations, the first 0),
+    the density change is the piecewise-constant-rate form of
+    :func:`_density_change_demographic`, in time scaled by ``2 Ne_ref``
+    (``demographic.py:146-208``).
 
     Departure from upstream, on purpose: where upstream's time grid leaves
     the density unresolved -- fewer than 8 grid points within a factor 1e3
@@ -122,6 +197,11 @@ def generate_flow_field(mean_min=1e-5, mean_max=1e2, cv_min=1e-2, cv_max=1.0,
     """
     from scipy.special import digamma, gammaln
 
+    demo = None
+    if Ne_values is not None:
+        if epoch_boundaries is None or len(epoch_boundaries) < len(Ne_values):
+            raise ValueError("epoch_boundaries needs one start time per epoch")
+        demo = _epochs(Ne_values, epoch_boundaries, Ne_ref)
     lm = np.linspace(np.log10(mean_min), np.log10(mean_max), mean_n)
     lc = np.linspace(np.log10(cv_min), np.log10(cv_max), cv_n)
     ln10 = np.log(10.0)
@@ -144,7 +224,10 @@ def generate_flow_field(mean_min=1e-5, mean_max=1e2, cv_min=1e-2, cv_max=1.0,
                 if peak > -700.0 - np.log(1e-12) and np.count_nonzero(log_pdf > peak - np.log(1e3)) >= 8:
                     break
             w = np.sqrt(np.append(np.diff(t), 0.0))
-            rhs = _density_change(t, alpha, beta) * w
+            with np.errstate(over="ignore", invalid="ignore"):
+                change = _density_change(t, alpha, beta) if demo is None \
+                    else _density_change_demographic(t, alpha, beta, *demo)
+            rhs = change * w
             pdf = np.exp(np.where(log_pdf > -700.0, log_pdf, -700.0))
             # d pdf / d log10(alpha) and d pdf / d log10(beta), weighted.
             d_la = pdf * (np.log(beta) - digamma(alpha) + np.log(tt)) * w * alpha / ln10
@@ -153,6 +236,8 @@ def generate_flow_field(mean_min=1e-5, mean_max=1e2, cv_min=1e-2, cv_max=1.0,
             if keep.sum() < 3:
                 continue
             design = np.column_stack([d_la, d_lb])[keep]
+            if not (np.isfinite(rhs[keep]).all() and np.isfinite(design).all()):
+                continue  # an overflowing integral: upstream's fit raises and keeps a zero flow
             try:
                 sol = np.linalg.lstsq(design, rhs[keep], rcond=None)[0]
             except np.linalg.LinAlgError:
@@ -161,6 +246,33 @@ def generate_flow_field(mean_min=1e-5, mean_max=1e2, cv_min=1e-2, cv_max=1.0,
             u[i, j] = sol[0] - sol[1]
             v[i, j] = -0.5 * sol[0]
     return u / CALIBRATION, v / CALIBRATION
+
+
+def estimate_demography(tmrca_means, Ne_init, n_epochs=20, t_max=None):
+    """Stage 1 of the two-stage demographic fit (``demographic.py:227-245``):
+    epochs with quadratically spaced boundaries on [0, t_max] generations
+    (default 10 Ne_init); in each epoch holding at least 10 posterior means,
+    N = max(100, survival / (2 density)) from the empirical density and
+    survival of the means; other epochs keep Ne_init. Returns
+    ``(Ne_values, boundaries)`` (``n_epochs`` sizes, ``n_epochs + 1``
+    boundaries)."""
+    if t_max is None:
+        t_max = 10.0 * Ne_init
+    vals = np.asarray(tmrca_means, np.float64).ravel()
+    vals = vals[np.isfinite(vals) & (vals > 0)]
+    bounds = t_max * np.linspace(0.0, 1.0, n_epochs + 1) ** 2
+    bounds[0] = 0.0
+    Ne = np.full(n_epochs, float(Ne_init))
+    for k in range(n_epochs):
+        lo, hi = bounds[k], bounds[k + 1]
+        count = np.count_nonzero((vals >= lo) & (vals < hi))
+        if count < 10 or hi <= lo:
+            continue
+        density = count / (vals.size * (hi - lo))
+        survival = np.mean(vals >= lo)
+        if survival > 0.01:
+            Ne[k] = max(100.0, 1.0 / (2.0 * density / survival))
+    return Ne, bounds
 
 
 def write_flow_field(path, u, v, mean_min=1e-5, mean_max=1e2, cv_min=1e-2, cv_max=1.0):

@@ -185,3 +185,31 @@ def test_fp64_tables_and_error_report(genotype_data, flow_field_path):
     for m in (m32, m64t):
         err = np.abs(np.log(m) - np.log(exact))
         assert err.max() <= 1e-2 and np.median(err) <= 1e-3
+
+
+def test_demographic_two_stage_context(genotype_data, flow_field_path):
+    """Two-stage demographic decoding: the stage-2 field is the FP64 field
+    of the estimated N(t), and decoding on it is bit-exact to the oracle
+    built from the same field file."""
+    from gamma_smc_cu_amd import flow_field as F
+    from gamma_smc_cu_amd.demographic import DemographicFlowContext
+
+    G, pos = genotype_data
+    mu, rho = O.estimate_scaled_params(G, pos, 1.25e-8, 1e-8, 1e4)
+    _case("demographic context")
+    d = DemographicFlowContext(G, pos, 1e4, mu, rho, flow_field_path, n_calibration_pairs=20, n_epochs=5,
+                               n_steps=100, verbose=False)
+    Ne, bounds = d.demography
+    assert Ne.shape == (5,) and bounds.shape == (6,) and np.all(Ne >= 100)
+    u, v, _ = F.read_flow_field(d.flow_field_path)
+    gu, gv = F.generate_flow_field(n_steps=100, Ne_values=Ne, epoch_boundaries=bounds, Ne_ref=1e4)
+    np.testing.assert_allclose(u, gu, atol=1e-10)
+    np.testing.assert_allclose(v, gv, atol=1e-10)
+    pairs = [(0, 1), (2, 3), (5, 4)]
+    r = d.run_fb(pairs)
+    T = O.tables_f32(d.flow_field_path, 1000, 1e4, mu, rho)
+    assert _eq_bits(r["mean"], O.fb_window_f32(G, pos, pairs, T, 1e4)["mean"])
+    s = d.run_fb_summary(pairs)
+    assert s["n_pairs"] == 3 and np.isfinite(s["site_mean"]).all()
+    f = d.run_fwd(pairs)
+    assert f["mean"].shape == r["mean"].shape and (f["mean"] > 0).all()
