@@ -33,6 +33,15 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per decode launch pair measured with rocprofv3 FETCH_SIZE and
+# WRITE_SIZE passes (scaled by the calibration copy kernel: x2.000 read,
+# x1.000 write), for one workload and launch shape; the summary with the
+# commands is profiles/fb_decode_chromosome_counters.md. Reported as
+# roofline.traffic only when a run matches that workload and shape.
+MEASURED_TRAFFIC = {
+    ("chromosome", 16, 1, "static", False): 151.49e9,
+}
+
 WORKLOADS = {
     "chromosome": dict(n=356, S=402_853, span=51_000_000, seed=7, flank=2048, fp64_pairs=256,
                        label="356 haplotypes (63,190 pairs) x 402,853 sites, 1/k spectrum, "
@@ -602,7 +611,10 @@ def main():
                    "core_block_sites": core, "flank_sites": flank, "n_blocks": n_blocks,
                    "tuning": tuning, "mode": "mean_only", "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": MEASURED_TRAFFIC.get((args.workload, tuning["checkpoint_k"],
+                                                      tuning["pairs_per_lane"], tuning["schedule"],
+                                                      bool(tuning["xcd_order"]))),
                      "kernel": ("fb_fwd_items + fb_bwd_items" if tuning["schedule"] == "static"
                                 else "fb_fwd_queue + fb_bwd_queue"),
                      "algorithmic_bytes_per_launch": B},
