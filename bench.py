@@ -422,6 +422,8 @@ def main():
                     help="take the launch shape from the config.tuning of a previous bench JSON line")
     ap.add_argument("--fp64-pairs", type=int, default=None,
                     help="pairs of the sample whose FP32 posterior is compared with the FP64 oracle")
+    ap.add_argument("--graph-steps", type=int, default=2,
+                    help="also time the decode captured into a HIP graph and replayed this many times")
     ap.add_argument("--codes-steps", type=int, default=2,
                     help="also time the decode with uint8 log-quantized output this many times")
     ap.add_argument("--fwd-steps", type=int, default=2,
@@ -550,6 +552,30 @@ def main():
                     "mode": "run_fwd, full sequence, mean only"}
         log(f"forward-only: {fm:.2f} ms/call")
 
+    graph = None
+    if args.graph_steps > 0:
+        # The same decode captured once into a HIP graph and replayed.
+        from gamma_smc_cu_amd._core import Job
+        from gamma_smc_cu_amd.batch import Graph
+
+        jb = (Job * 1)()
+        jb[0].site_lo, jb[0].site_hi, jb[0].core_block_sites, jb[0].flank_sites = 0, S, core, flank
+        jb[0].pair_i, jb[0].pair_j, jb[0].n_pairs, jb[0].mean = d_pi.ptr.value, d_pj.ptr.value, P, d_out.ptr.value
+        gr = Graph(ctx, jb, 1, 0)
+        gr.launch(stream)
+        stream.synchronize()
+        e0, e1 = Event(), Event()
+        e0.record(stream)
+        for _ in range(args.graph_steps):
+            gr.launch(stream)
+        e1.record(stream)
+        stream.synchronize()
+        gm = e0.elapsed_ms(e1) / args.graph_steps
+        graph = {"ms": gm, "pair_sites_per_s": pair_sites / (gm / 1e3), "steps": args.graph_steps,
+                 "mode": "captured decode replayed from a HIP graph"}
+        log(f"graph replay: {gm:.2f} ms/step")
+        gr.close()
+
     codes = None
     if args.codes_steps > 0:
         # The same decode with the mean written as uint8 log codes (1 B per
@@ -626,6 +652,7 @@ def main():
         "site_summary": summary,
         "forward_only": fwd_only,
         "codes_u8": codes,
+        "graph_replay": graph,
         "checks": checks,
         "wall_ms_per_step": wall * 1e3 / args.steps,
         "copy_calibration_bytes": copy_bytes,

@@ -43,7 +43,8 @@ EXPORTS = (
     "gsmc_event_elapsed_ms", "gsmc_batch_create", "gsmc_batch_destroy", "gsmc_batch_launch", "gsmc_batch_wait",
     "gsmc_batch_query", "gsmc_checkpoint_bytes", "gsmc_device_copy", "gsmc_memcpy_htod_async",
     "gsmc_memcpy_dtoh_async", "gsmc_host_alloc", "gsmc_host_free", "gsmc_event_query",
-    "gsmc_batch_reserve", "gsmc_run_fb_summary", "gsmc_run_fwd", "gsmc_run_fwd_device",
+    "gsmc_batch_reserve", "gsmc_run_fb_summary", "gsmc_run_fwd", "gsmc_run_fwd_device", "gsmc_batch_capture",
+    "gsmc_graph_launch", "gsmc_graph_destroy",
 )
 
 
@@ -114,6 +115,8 @@ def _declare(L):
         "gsmc_run_fb_summary": [_vp, _vp, _vp, _i64, _i64, _i64, _vp, _vp, _vp],
         "gsmc_run_fwd": [_vp, _vp, _vp, _i64, _i32, _vp, _vp, _vp],
         "gsmc_run_fwd_device": [_vp, _vp, _vp, _i64, _i32, _vp, _vp, _vp, _vp],
+        "gsmc_batch_capture": [_vp, _vp, _i64, _i32, ctypes.POINTER(_vp)],
+        "gsmc_graph_launch": [_vp, _vp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -123,6 +126,8 @@ def _declare(L):
     L.gsmc_context_destroy.restype = None
     L.gsmc_batch_destroy.argtypes = [_vp]
     L.gsmc_batch_destroy.restype = None
+    L.gsmc_graph_destroy.argtypes = [_vp]
+    L.gsmc_graph_destroy.restype = None
 
 
 def lib():

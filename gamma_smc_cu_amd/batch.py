@@ -73,6 +73,33 @@ class Workspace:
             pass
 
 
+class Graph:
+    """A batched launch captured into a HIP graph (``gsmc_batch_capture``):
+    planned and uploaded once, replayed with :meth:`launch` on a stream.
+    Replays write the outputs the captured jobs point to; replays of one
+    graph must not overlap."""
+
+    def __init__(self, ctx, jobs_struct, n_jobs, flags=0):
+        h = ctypes.c_void_p()
+        _core.check(_core.lib().gsmc_batch_capture(ctx.handle, jobs_struct, int(n_jobs), int(flags), ctypes.byref(h)))
+        self.handle = h
+        self.ctx = ctx
+
+    def launch(self, stream=None):
+        _core.check(_core.lib().gsmc_graph_launch(self.handle, None if stream is None else stream.handle))
+
+    def close(self):
+        if self.handle is not None:
+            _core.lib().gsmc_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def _validate(ctx, jobs):
     for k, j in enumerate(jobs):
         if not (0 <= j.site_lo <= j.site_hi <= ctx.n_sites):

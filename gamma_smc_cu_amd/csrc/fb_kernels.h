@@ -27,8 +27,13 @@
 namespace gsmc {
 
 constexpr int kGridNodes = 2550;
-constexpr uint32_t kSlabBytes = kGridNodes * 8;      // one [2550] (m, c) float2 slab
-constexpr uint32_t kLayerBytes = 2 * kSlabBytes;     // hom slab then het slab
+// Table slabs are stored by cell row pairs: entry (r, q), r < 50, q < 50,
+// is the 16-byte {m, c at node (r, q); m, c at node (r + 1, q)}, so the four
+// corners of cell (r, q) are the 32 contiguous bytes of entries (r, q) and
+// (r, q + 1).
+constexpr uint32_t kPairRows = 50;
+constexpr uint32_t kSlabBytes = kPairRows * 50 * 16;  // one slab: 40,000 bytes
+constexpr uint32_t kLayerBytes = 2 * kSlabBytes;      // hom slab then het slab
 constexpr uint32_t kDescLayerBits = 13;              // site descriptor: layer | chunks << 13
 constexpr uint32_t kDescLayerMask = (1u << kDescLayerBits) - 1;
 
@@ -128,14 +133,36 @@ struct FbArgs {
     float* pmax;               // ... their maximum
 };
 
+// Lane v of this lane's quad (DPP quad permutation; every lane active).
+template <int CTRL> __device__ __forceinline__ uint32_t quad_perm(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL> __device__ __forceinline__ float quad_perm(float v)
+{
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+constexpr int kQuad0101 = 0x44;  // lanes read lanes 0, 1, 0, 1 of their quad
+constexpr int kQuad2323 = 0xEE;  // 2, 3, 2, 3
+constexpr int kQuad2301 = 0x4E;  // 2, 3, 0, 1
+
 // Bilinear lookup of one table slab at (m, c); updates (m, c). het (0 or
-// 1) selects the hom or het slab of the layer.
+// 1) selects the hom or het slab of the layer. Every lane of the wave must
+// call it together.
 //
 // Same values as gamma_smc_flow.cu:408-430 in fewer instructions: the clamp
 // is one med3 (identical for any non-NaN index), the cell corner is
 // floor(index) capped at the last cell (identical to truncating a
 // non-negative index and decrementing 50 -> 49 / 49 -> 48), and the byte
-// offset of the corner is formed in float, exact for these small integers.
+// offset of the corner entry is formed in float, exact for these small
+// integers.
+//
+// Loads: the texture path handles a 16-byte gather four lanes at a time and
+// pays one cache access per distinct line in each such quad. Each lane needs
+// 32 contiguous bytes, so the quad's first gather fetches both halves for
+// lanes 0 and 1 and the second both halves for lanes 2 and 3 (two lines per
+// quad and gather instead of up to four), and a quad permutation hands every
+// lane its own corners.
 __device__ __forceinline__ void lookup(const char* __restrict__ layer, uint32_t het, float& m, float& c)
 {
     float fm = (m - -5.0f) * kMeanStepInv;
@@ -146,14 +173,25 @@ __device__ __forceinline__ void lookup(const char* __restrict__ layer, uint32_t 
     const float lc = fminf(floorf(fc), 48.0f);
     const float wm = fm - lm;
     const float wc = fc - lc;
-    const uint32_t off = __umul24(het, kSlabBytes) + (uint32_t)fmaf(lm, 400.0f, lc * 8.0f);
-    const float2 v00 = *reinterpret_cast<const float2*>(layer + off);
-    const float2 v01 = *reinterpret_cast<const float2*>(layer + off + 8);
-    const float2 v10 = *reinterpret_cast<const float2*>(layer + off + 400);
-    const float2 v11 = *reinterpret_cast<const float2*>(layer + off + 408);
+    const uint32_t off = __umul24(het, kSlabBytes) + (uint32_t)fmaf(lm, 800.0f, lc * 16.0f);
+    const bool upper = (__lane_id() & 2u) != 0;
+    // first gather: lanes 0, 1 their entry (r, q); lanes 2, 3 entry (r, q + 1) of lanes 0, 1
+    const uint32_t a1 = quad_perm<kQuad0101>(off) + (upper ? 16u : 0u);
+    // second gather: lanes 2, 3 their entry (r, q); lanes 0, 1 entry (r, q + 1) of lanes 2, 3
+    const uint32_t a2 = quad_perm<kQuad2323>(off) + (upper ? 0u : 16u);
+    const float4 x = *reinterpret_cast<const float4*>(layer + a1);
+    const float4 y = *reinterpret_cast<const float4*>(layer + a2);
+    // own entry (r, q) = {v00, v10}; own entry (r, q + 1) = {v01, v11}
+    const float4 e0 = upper ? y : x;
+    const float4 z = upper ? x : y;
+    float4 e1;
+    e1.x = quad_perm<kQuad2301>(z.x);
+    e1.y = quad_perm<kQuad2301>(z.y);
+    e1.z = quad_perm<kQuad2301>(z.z);
+    e1.w = quad_perm<kQuad2301>(z.w);
     const float omwm = 1.0f - wm, omwc = 1.0f - wc;
-    m = fmaf(wm, fmaf(wc, v11.x, omwc * v10.x), omwm * fmaf(wc, v01.x, omwc * v00.x));
-    c = fmaf(wm, fmaf(wc, v11.y, omwc * v10.y), omwm * fmaf(wc, v01.y, omwc * v00.y));
+    m = fmaf(wm, fmaf(wc, e1.z, omwc * e0.z), omwm * fmaf(wc, e1.x, omwc * e0.x));
+    c = fmaf(wm, fmaf(wc, e1.w, omwc * e0.w), omwm * fmaf(wc, e1.y, omwc * e0.y));
 }
 
 __device__ __forceinline__ const char* layer_of(const char* base, uint32_t d)
@@ -683,6 +721,10 @@ __device__ __forceinline__ void backward_item(const FbArgs& a, int item, int lan
     }
 }
 
+// Waves per SIMD the backward kernel is built for: three with one pair per
+// lane (a few registers over would leave two), no floor with two.
+template <int NP> constexpr int kBwdWaves = NP == 1 ? 3 : 1;
+
 // A decode is two launches over the same work items: fb_fwd_* (forward
 // walk + right-flank backward walk; few registers, so many waves per SIMD
 // to hide the latency of its single lookup chain) then fb_bwd_* (the
@@ -697,7 +739,7 @@ __global__ __launch_bounds__(64) void fb_fwd_items(FbArgs a)
 }
 
 template <int K, int NP, bool CI, bool POST, bool SUM>
-__global__ __launch_bounds__(64) void fb_bwd_items(FbArgs a)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kBwdWaves<NP>))) void fb_bwd_items(FbArgs a)
 {
     const int item = __builtin_amdgcn_readfirstlane((int)blockIdx.x);
     __shared__ float red[SUM ? 16 * 65 : 1];
@@ -719,7 +761,7 @@ __global__ __launch_bounds__(64) void fb_fwd_queue(FbArgs a)
 }
 
 template <int K, int NP, bool CI, bool POST, bool SUM>
-__global__ __launch_bounds__(64) void fb_bwd_queue(FbArgs a)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kBwdWaves<NP>))) void fb_bwd_queue(FbArgs a)
 {
     __shared__ float red[SUM ? 16 * 65 : 1];
     for (int item = next_item(a.queue2, threadIdx.x); item < a.n_items; item = next_item(a.queue2, threadIdx.x))
@@ -761,14 +803,17 @@ __global__ __launch_bounds__(256) void summary_reduce_kernel(const double* __res
 
 // prior[q] for q < n: q applications of the N-bp hom stretch to the prior
 // (0, 0), one dependent lookup each, with the decode kernels' own lookup.
+// (one wave: every lane walks the same chain, lookups need the whole wave;
+// lane 0 stores)
 __global__ void prior_table_kernel(DevTables t, float2* out, int64_t n)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (blockIdx.x != 0) return;
+    const bool writer = threadIdx.x == 0;
     float m = 0.0f, c = 0.0f;
-    out[0] = make_float2(m, c);
+    if (writer) out[0] = make_float2(m, c);
     for (int64_t q = 1; q < n; ++q) {
         lookup(t.stretch, 0u, m, c);
-        out[q] = make_float2(m, c);
+        if (writer) out[q] = make_float2(m, c);
     }
 }
 

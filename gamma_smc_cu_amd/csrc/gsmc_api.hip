@@ -187,9 +187,27 @@ std::shared_ptr<DeviceTables> get_tables(int device, const char* path, int N, do
     const size_t layers = (size_t)N * kLayerBytes;
     dt->bytes = 2 * layers + kSlabBytes;
     HIP_OK(hipMalloc(&dt->base, dt->bytes));
-    HIP_OK(hipMemcpy(dt->base, host.fwd.data(), layers, hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(dt->base + layers, host.bwd.data(), layers, hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(dt->base + 2 * layers, host.stretch_last.data(), kSlabBytes, hipMemcpyHostToDevice));
+    {
+        // Node-major (m, c) slabs -> row-pair entries (see kSlabBytes).
+        std::vector<float> dev(dt->bytes / sizeof(float));
+        auto slab = [&](const float* nodes, float* out) {
+            for (uint32_t r = 0; r < kPairRows; ++r)
+                for (uint32_t q = 0; q < 50; ++q) {
+                    float* e = out + (size_t)(r * 50 + q) * 4;
+                    e[0] = nodes[2 * (r * 50 + q)];
+                    e[1] = nodes[2 * (r * 50 + q) + 1];
+                    e[2] = nodes[2 * ((r + 1) * 50 + q)];
+                    e[3] = nodes[2 * ((r + 1) * 50 + q) + 1];
+                }
+        };
+        const size_t slab_f = kSlabBytes / sizeof(float), node_f = (size_t)kGridNodes * 2;
+        for (size_t k = 0; k < (size_t)N * 2; ++k) {
+            slab(host.fwd.data() + k * node_f, dev.data() + k * slab_f);
+            slab(host.bwd.data() + k * node_f, dev.data() + layers / sizeof(float) + k * slab_f);
+        }
+        slab(host.stretch_last.data(), dev.data() + 2 * layers / sizeof(float));
+        HIP_OK(hipMemcpy(dt->base, dev.data(), dt->bytes, hipMemcpyHostToDevice));
+    }
     dt->view.fwd = dt->base;
     dt->view.bwd = dt->base + layers;
     dt->view.stretch = dt->base + 2 * layers;
@@ -611,28 +629,22 @@ void launch(const void* fn, unsigned grid, const FbArgs& a, hipStream_t st)
     HIP_OK(hipLaunchKernel(fn, dim3(grid), dim3(64), args, 0, st));
 }
 
-// Queue one launch of `jobs` on `stream` using workspace b.
-void launch_plan(gsmc_batch* b, const std::vector<JobSpec>& jobs, int flags, hipStream_t stream)
+// Buffers a planned launch needs on workspace b.
+void ensure_buffers(gsmc_batch* b, const Plan& pl, bool summary)
 {
     gsmc_context* ctx = b->ctx;
-    const Tuning tu = ctx->tune;
-    const bool summary = (flags & GSMC_SUMMARY) != 0;
-    Plan pl = make_plan(ctx, jobs, tu, launch_np(tu, flags), summary);
-    if (pl.n_items == 0) return;
-    const bool same = b->pending && stream == b->last_stream && pl.bytes == b->last;
-    if (!same) {
-        if (b->pending) HIP_OK(hipEventSynchronize(b->done));
-        b->pending = false;
-        b->meta.ensure(pl.bytes.size(), ctx->device);
-        b->ck.ensure(std::max<size_t>(pl.ck_elems, 1) * sizeof(float2), ctx->device);
-        b->queue.ensure(256, ctx->device);
-        b->yb.ensure((size_t)std::max<int64_t>(pl.n_items, 1) * 64 * 2 * sizeof(float2), ctx->device);
-        b->stage.ensure(pl.bytes.size());
-        std::memcpy(b->stage.p, pl.bytes.data(), pl.bytes.size());
-        HIP_OK(hipMemcpyAsync(b->meta.p, b->stage.p, pl.bytes.size(), hipMemcpyHostToDevice, stream));
-        b->last.swap(pl.bytes);
-        b->last_stream = stream;
-    }
+    b->meta.ensure(pl.bytes.size(), ctx->device);
+    b->ck.ensure(std::max<size_t>(pl.ck_elems, 1) * sizeof(float2), ctx->device);
+    b->queue.ensure(256, ctx->device);
+    b->yb.ensure((size_t)std::max<int64_t>(pl.n_items, 1) * 64 * 2 * sizeof(float2), ctx->device);
+    if (summary)
+        b->part.ensure(std::max<size_t>(pl.part_elems, 1) * (sizeof(double) + 2 * sizeof(float)), ctx->device);
+}
+
+// Kernel arguments of a planned launch on workspace b's buffers.
+FbArgs make_args(gsmc_batch* b, const Plan& pl, int flags)
+{
+    gsmc_context* ctx = b->ctx;
     FbArgs a;
     a.packed = ctx->packed.as<uint64_t>();
     a.n_haps = ctx->n;
@@ -653,12 +665,21 @@ void launch_plan(gsmc_batch* b, const std::vector<JobSpec>& jobs, int flags, hip
     a.unscale = 2.0f * ctx->Ne;
     a.psum = nullptr;
     a.pmin = a.pmax = nullptr;
-    if (summary) {
-        b->part.ensure(std::max<size_t>(pl.part_elems, 1) * (sizeof(double) + 2 * sizeof(float)), ctx->device);
+    if (flags & GSMC_SUMMARY) {
+        const size_t n = std::max<size_t>(pl.part_elems, 1);
         a.psum = b->part.as<double>();
-        a.pmin = reinterpret_cast<float*>(a.psum + std::max<size_t>(pl.part_elems, 1));
-        a.pmax = a.pmin + std::max<size_t>(pl.part_elems, 1);
+        a.pmin = reinterpret_cast<float*>(a.psum + n);
+        a.pmax = a.pmin + n;
     }
+    return a;
+}
+
+// The kernels of a planned launch, in order: the queue reset (persistent
+// schedule), the two decode kernels, the summary epilogues. No copies and
+// no memsets, so the sequence can also be captured into a graph.
+void enqueue_kernels(gsmc_context* ctx, const Plan& pl, const FbArgs& a, int flags, hipStream_t stream)
+{
+    const Tuning& tu = ctx->tune;
     const KernelPair kp = kernels_for(tu, flags);
     if (tu.schedule == 1) {
         hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(1), 0, stream, a.queue);
@@ -677,6 +698,29 @@ void launch_plan(gsmc_batch* b, const std::vector<JobSpec>& jobs, int flags, hip
                            js.out.site_mean, js.out.site_min, js.out.site_max);
     }
     HIP_OK(hipGetLastError());
+}
+
+// Queue one launch of `jobs` on `stream` using workspace b.
+void launch_plan(gsmc_batch* b, const std::vector<JobSpec>& jobs, int flags, hipStream_t stream)
+{
+    gsmc_context* ctx = b->ctx;
+    const Tuning tu = ctx->tune;
+    const bool summary = (flags & GSMC_SUMMARY) != 0;
+    Plan pl = make_plan(ctx, jobs, tu, launch_np(tu, flags), summary);
+    if (pl.n_items == 0) return;
+    const bool same = b->pending && stream == b->last_stream && pl.bytes == b->last;
+    if (!same) {
+        if (b->pending) HIP_OK(hipEventSynchronize(b->done));
+        b->pending = false;
+        ensure_buffers(b, pl, summary);
+        b->stage.ensure(pl.bytes.size());
+        std::memcpy(b->stage.p, pl.bytes.data(), pl.bytes.size());
+        HIP_OK(hipMemcpyAsync(b->meta.p, b->stage.p, pl.bytes.size(), hipMemcpyHostToDevice, stream));
+        b->last.swap(pl.bytes);
+        b->last_stream = stream;
+    }
+    const FbArgs a = make_args(b, pl, flags);
+    enqueue_kernels(ctx, pl, a, flags, stream);
     HIP_OK(hipEventRecord(b->done, stream));
     b->pending = true;
 }
@@ -1321,6 +1365,117 @@ int gsmc_batch_reserve(gsmc_batch* b, uint64_t checkpoint_bytes, int64_t max_ite
     });
 }
 
+namespace gsmc {
+namespace {
+std::vector<JobSpec> specs_of(const gsmc_context* ctx, const gsmc_job* jobs, int64_t n_jobs, int flags);
+}
+}  // namespace gsmc
+
+// A captured launch: private buffers with the plan's metadata uploaded, and
+// an executable graph of the launch's kernels.
+struct gsmc_graph {
+    gsmc_context* ctx = nullptr;
+    std::unique_ptr<gsmc_batch> buf;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~gsmc_graph()
+    {
+        if (ctx) (void)hipSetDevice(ctx->device);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
+int gsmc_batch_capture(gsmc_context* ctx, const gsmc_job* jobs, int64_t n_jobs, int flags, gsmc_graph** out)
+{
+    return guarded([&] {
+        require(ctx != nullptr && out != nullptr, "NULL argument");
+        require(n_jobs >= 0 && (jobs != nullptr || n_jobs == 0), "bad job list");
+        require(!((flags & GSMC_SUMMARY) && (flags & (GSMC_CI | GSMC_POSTERIOR))),
+                "GSMC_SUMMARY cannot be combined with GSMC_CI or GSMC_POSTERIOR");
+        std::vector<JobSpec> specs = specs_of(ctx, jobs, n_jobs, flags);
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        HIP_OK(hipSetDevice(ctx->device));
+        auto g = std::make_unique<gsmc_graph>();
+        g->ctx = ctx;
+        g->buf = new_batch(ctx);
+        const bool summary = (flags & GSMC_SUMMARY) != 0;
+        Plan pl = make_plan(ctx, specs, ctx->tune, launch_np(ctx->tune, flags), summary);
+        require(pl.n_items > 0, "nothing to capture: every job is empty");
+        ensure_buffers(g->buf.get(), pl, summary);
+        HIP_OK(hipMemcpy(g->buf->meta.p, pl.bytes.data(), pl.bytes.size(), hipMemcpyHostToDevice));
+        const FbArgs a = make_args(g->buf.get(), pl, flags);
+        hipStream_t s = nullptr;
+        HIP_OK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+            enqueue_kernels(ctx, pl, a, flags, s);
+        } catch (...) {
+            hipGraph_t partial = nullptr;
+            (void)hipStreamEndCapture(s, &partial);
+            if (partial) (void)hipGraphDestroy(partial);
+            (void)hipStreamDestroy(s);
+            throw;
+        }
+        const hipError_t e = hipStreamEndCapture(s, &g->graph);
+        (void)hipStreamDestroy(s);
+        HIP_OK(e);
+        HIP_OK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+        *out = g.release();
+    });
+}
+
+int gsmc_graph_launch(gsmc_graph* g, void* stream)
+{
+    return guarded([&] {
+        require(g != nullptr, "graph is NULL");
+        HIP_OK(hipSetDevice(g->ctx->device));
+        HIP_OK(hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream)));
+    });
+}
+
+void gsmc_graph_destroy(gsmc_graph* g) { delete g; }
+
+namespace gsmc {
+namespace {
+std::vector<JobSpec> specs_of(const gsmc_context* ctx, const gsmc_job* jobs, int64_t n_jobs, int flags)
+{
+    std::vector<JobSpec> specs;
+    specs.reserve((size_t)n_jobs);
+    for (int64_t i = 0; i < n_jobs; ++i) {
+        const gsmc_job& j = jobs[i];
+        require(j.site_lo >= 0 && j.site_lo <= j.site_hi && j.site_hi <= ctx->S, "job site range out of bounds");
+        require(j.n_pairs >= 0, "negative n_pairs");
+        require(j.flank_sites >= 0, "flank_sites must be non-negative.");
+        if (j.n_pairs == 0 || j.site_hi == j.site_lo) continue;
+        require(j.pair_i != nullptr && j.pair_j != nullptr, "job pointers are NULL");
+        require((flags & GSMC_SUMMARY) ? j.site_mean != nullptr : j.mean != nullptr, "job outputs are NULL");
+        require(!(flags & GSMC_CI) || (j.lower && j.upper), "lower/upper required with GSMC_CI");
+        require(!(flags & GSMC_POSTERIOR) || (j.alpha && j.beta), "alpha/beta required with GSMC_POSTERIOR");
+        const int64_t len = j.site_hi - j.site_lo;
+        JobSpec js;
+        js.seq_lo = j.site_lo;
+        js.blocks = make_blocks(len, j.core_block_sites > 0 ? j.core_block_sites : len,
+                                j.core_block_sites > 0 ? j.flank_sites : 0);
+        for (Block& bk : js.blocks) {
+            bk.cs += (int32_t)j.site_lo;
+            bk.ce += (int32_t)j.site_lo;
+            bk.ps += (int32_t)j.site_lo;
+            bk.pe += (int32_t)j.site_lo;
+        }
+        js.pi = j.pair_i;
+        js.pj = j.pair_j;
+        js.P = j.n_pairs;
+        js.out = Outputs{j.mean, j.lower, j.upper, j.alpha, j.beta, j.site_mean, j.site_min, j.site_max};
+        js.row0 = j.site_lo;
+        check_codes(flags, js.blocks, js.row0);
+        specs.push_back(std::move(js));
+    }
+    return specs;
+}
+}  // namespace
+}  // namespace gsmc
+
 int gsmc_batch_launch(gsmc_batch* b, const gsmc_job* jobs, int64_t n_jobs, int flags, void* stream)
 {
     return guarded([&] {
@@ -1329,37 +1484,7 @@ int gsmc_batch_launch(gsmc_batch* b, const gsmc_job* jobs, int64_t n_jobs, int f
         gsmc_context* ctx = b->ctx;
         require(!((flags & GSMC_SUMMARY) && (flags & (GSMC_CI | GSMC_POSTERIOR))),
                 "GSMC_SUMMARY cannot be combined with GSMC_CI or GSMC_POSTERIOR");
-        std::vector<JobSpec> specs;
-        specs.reserve((size_t)n_jobs);
-        for (int64_t i = 0; i < n_jobs; ++i) {
-            const gsmc_job& j = jobs[i];
-            require(j.site_lo >= 0 && j.site_lo <= j.site_hi && j.site_hi <= ctx->S, "job site range out of bounds");
-            require(j.n_pairs >= 0, "negative n_pairs");
-            require(j.flank_sites >= 0, "flank_sites must be non-negative.");
-            if (j.n_pairs == 0 || j.site_hi == j.site_lo) continue;
-            require(j.pair_i != nullptr && j.pair_j != nullptr, "job pointers are NULL");
-            require((flags & GSMC_SUMMARY) ? j.site_mean != nullptr : j.mean != nullptr, "job outputs are NULL");
-            require(!(flags & GSMC_CI) || (j.lower && j.upper), "lower/upper required with GSMC_CI");
-            require(!(flags & GSMC_POSTERIOR) || (j.alpha && j.beta), "alpha/beta required with GSMC_POSTERIOR");
-            const int64_t len = j.site_hi - j.site_lo;
-            JobSpec js;
-            js.seq_lo = j.site_lo;
-            js.blocks = make_blocks(len, j.core_block_sites > 0 ? j.core_block_sites : len,
-                                    j.core_block_sites > 0 ? j.flank_sites : 0);
-            for (Block& bk : js.blocks) {
-                bk.cs += (int32_t)j.site_lo;
-                bk.ce += (int32_t)j.site_lo;
-                bk.ps += (int32_t)j.site_lo;
-                bk.pe += (int32_t)j.site_lo;
-            }
-            js.pi = j.pair_i;
-            js.pj = j.pair_j;
-            js.P = j.n_pairs;
-            js.out = Outputs{j.mean, j.lower, j.upper, j.alpha, j.beta, j.site_mean, j.site_min, j.site_max};
-            js.row0 = j.site_lo;
-            check_codes(flags, js.blocks, js.row0);
-            specs.push_back(std::move(js));
-        }
+        std::vector<JobSpec> specs = specs_of(ctx, jobs, n_jobs, flags);
         std::lock_guard<std::mutex> lk(ctx->mu);
         HIP_OK(hipSetDevice(ctx->device));
         launch_plan(b, specs, flags, static_cast<hipStream_t>(stream));

@@ -43,6 +43,7 @@ extern "C" {
 
 typedef struct gsmc_context gsmc_context;
 typedef struct gsmc_batch gsmc_batch;
+typedef struct gsmc_graph gsmc_graph;
 
 /* One job of a batched launch: pairs decoded over sites [site_lo, site_hi)
  * of the context as one sequence (its first site's gap measured from
@@ -216,6 +217,18 @@ void gsmc_batch_destroy(gsmc_batch* batch);
 int gsmc_batch_launch(gsmc_batch* batch, const gsmc_job* jobs, int64_t n_jobs, int flags, void* stream);
 int gsmc_batch_wait(gsmc_batch* batch);
 int gsmc_batch_query(gsmc_batch* batch, int* done);
+/* Capture a batched launch of `jobs` into an executable HIP graph with its
+ * own launch buffers: the plan and its metadata upload happen once, here,
+ * and each gsmc_graph_launch replays only the launch's kernels (queue reset,
+ * decode, summary epilogue; no copies or memsets) on `stream`. The jobs'
+ * pair ids and outputs are read and written at replay time through the
+ * pointers given now. Replays of one graph must not overlap (they share its
+ * buffers). No reference counterpart (the reference launches every block
+ * from the host). */
+int gsmc_batch_capture(gsmc_context* ctx, const gsmc_job* jobs, int64_t n_jobs, int flags, gsmc_graph** out);
+int gsmc_graph_launch(gsmc_graph* graph, void* stream);
+void gsmc_graph_destroy(gsmc_graph* graph);
+
 /* Grow a batch's buffers now (checkpoint bytes, work items and jobs of the
  * largest launch the caller will make), so launches within these bounds
  * never allocate or synchronise the device. */

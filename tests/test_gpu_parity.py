@@ -392,3 +392,37 @@ def test_log_codes_match_oracle(core, cohort_small, flow_field_path, bits):
     if bits == 4:
         with pytest.raises(ValueError, match="even core_block_sites"):
             ctx.run_fb_codes(pairs, bits=4, core_block_sites=201, flank_sites=10)
+
+
+@pytest.mark.parametrize("schedule", ["static", "persistent"])
+def test_graph_replays_match_eager(core, cohort_small, flow_field_path, schedule):
+    """A captured batched launch, replayed several times with other eager
+    launches and synchronisations in between, reproduces the eager launch
+    bit for bit on every replay."""
+    from gamma_smc_cu_amd.batch import Graph, Staged, Workspace
+    from gamma_smc_cu_amd.device import Stream
+
+    G, pos = cohort_small
+    ctx = core.FlowContext(G, pos, 1e4, 1.25e-8, 1e-8, flow_field_path, 0)
+    ctx.set_tuning(checkpoint_k=16, pairs_per_lane=1, schedule=schedule)
+    jobs = [j for j in _ragged_jobs(G.shape[0], G.shape[1], np.random.default_rng(5)) if j.n_sites > 0]
+    st = Staged(jobs)
+    arr, n = st.structs()
+    ws = Workspace(ctx)
+    _case(f"graph eager reference ({schedule})")
+    ws.launch(arr, n, st.flags)
+    ws.wait()
+    eager = st.d_out.to_host().copy()
+    g = Graph(ctx, arr, n, st.flags)
+    other = Staged(jobs[:3])
+    oarr, on = other.structs()
+    s = Stream()
+    for r in range(4):
+        _case(f"graph replay {r} ({schedule})")
+        st.d_out.fill_zero()
+        g.launch(s)
+        s.synchronize()
+        assert _bits_equal(st.d_out.to_host(), eager), r
+        ws.launch(oarr, on, other.flags)
+        ws.wait()
+    g.close()
