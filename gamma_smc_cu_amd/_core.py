@@ -44,7 +44,7 @@ EXPORTS = (
     "gsmc_batch_query", "gsmc_checkpoint_bytes", "gsmc_device_copy", "gsmc_memcpy_htod_async",
     "gsmc_memcpy_dtoh_async", "gsmc_host_alloc", "gsmc_host_free", "gsmc_event_query",
     "gsmc_batch_reserve", "gsmc_run_fb_summary", "gsmc_run_fwd", "gsmc_run_fwd_device", "gsmc_batch_capture",
-    "gsmc_graph_launch", "gsmc_graph_destroy",
+    "gsmc_graph_launch", "gsmc_graph_destroy", "gsmc_context_set_state_budget",
 )
 
 
@@ -83,6 +83,7 @@ def _declare(L):
         "gsmc_context_info": [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(ctypes.c_int),
                               ctypes.POINTER(ctypes.c_float)],
         "gsmc_context_set_tuning": [_vp, _i32, _i32, _i32, _i32, _i64],
+        "gsmc_context_set_state_budget": [_vp, ctypes.c_uint64],
         "gsmc_run_fb": [_vp, _vp, _vp, _i64, _i32, _vp, _vp, _vp, _vp, _vp],
         "gsmc_block_windows": [_i64, _i64, _i64, _vp, _i64, ctypes.POINTER(_i64)],
         "gsmc_run_fb_blockwise": [_vp, _vp, _vp, _i64, _i64, _i64, _i64, _i32, _i32, _vp, _vp, _vp, _vp, _vp],
@@ -332,10 +333,19 @@ class FlowContext:
 
     def set_tuning(self, checkpoint_k=16, pairs_per_lane=1, schedule="static", xcd_order=False,
                    min_waves_per_launch=8192):
-        """Launch shape (results are identical under every setting)."""
+        """Launch shape (results are identical under every setting).
+        ``checkpoint_k=1`` stores the forward state at every core site
+        (8 bytes per pair and site of device memory) instead of
+        recomputing it in the backward kernel."""
         sched = {"static": 0, "persistent": 1}[schedule]
         check(lib().gsmc_context_set_tuning(self._h, int(checkpoint_k), int(pairs_per_lane), sched,
                                             int(bool(xcd_order)), int(min_waves_per_launch)))
+
+    def set_state_budget(self, max_bytes=0):
+        """Cap the device memory a launch uses for checkpoints / stored
+        forward states (0: free memory less 2 GiB); larger launches run in
+        consecutive phases. Results are identical under every setting."""
+        check(lib().gsmc_context_set_state_budget(self._h, int(max_bytes)))
 
     def run_fb_codes(self, pairs, bits=8, core_block_sites=0, flank_sites=0):
         """Blockwise decode whose posterior mean comes back as log-quantized

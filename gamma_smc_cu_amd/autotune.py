@@ -6,7 +6,10 @@ many sites a checkpoint window spans), so the tuner is free to pick the
 fastest. Candidates sweep:
 
 * ``checkpoint_k`` -- sites per checkpoint window, which is also the length
-  of the forward window recomputed into registers (8 or 16);
+  of the forward window recomputed into registers (8 or 16), or 1: every
+  forward state stored by the forward kernel and read back by the backward
+  kernel, one lookup per site fewer for 16 bytes of HBM traffic per
+  pair-site;
 * ``pairs_per_lane`` -- 1 or 2 pairs per lane, i.e. 64 or 128 pairs per
   64-lane workgroup (two pairs give every phase two independent lookup
   chains per lane, at the cost of registers);
@@ -27,7 +30,7 @@ CANDIDATES = [
     dict(checkpoint_k=k, pairs_per_lane=np_, schedule=sch, xcd_order=xcd)
     for k, np_, (sch, xcd) in itertools.product(
         (8, 16), (1, 2), (("static", False), ("static", True), ("persistent", False)))
-]
+] + [dict(checkpoint_k=1, pairs_per_lane=1, schedule=sch, xcd_order=False) for sch in ("static", "persistent")]
 
 
 def tune(ctx, run_once, candidates=None, repeats=1, log=None):

@@ -98,16 +98,19 @@ struct BlockDev {
     int32_t cs, ce, ps, pe;
     uint32_t sd_fwd, sd_bwd;
     int32_t job;
-    int32_t nwin;    // checkpoint windows of the core
-    int64_t ck_off;  // float2 offset of this block's checkpoints (tile-major)
+    int32_t nwin;    // K-site windows of the core
     int32_t q0;      // the walk starts from prior[q0]: the prior after q0 full N-bp hom stretches
     int32_t pad;
 };
 
-// A work item: one tile (64 pairs per lane slot) of one block window.
+// A work item: one tile (64 pairs per lane slot) of one block window, and
+// the float2 offset of its rows in the state buffer: one row of 64 NP
+// states per core window (checkpoints) or per core site (stored forward
+// states).
 struct ItemDev {
     int32_t block;
     int32_t tile;
+    int64_t ck_off;
 };
 
 struct FbArgs {
@@ -133,22 +136,25 @@ struct FbArgs {
     float* pmax;               // ... their maximum
 };
 
-// Lane v of this lane's quad (DPP quad permutation; every lane active).
+// Lane v of this lane's quad (DPP quad permutation; every lane active). The
+// zero "old" value with bound_ctrl lets the compiler fold the permutation
+// into the consuming add or multiply (one VOP2 DPP instruction instead of a
+// move and the operation).
 template <int CTRL> __device__ __forceinline__ uint32_t quad_perm(uint32_t v)
 {
-    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, false);
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, true);
 }
 template <int CTRL> __device__ __forceinline__ float quad_perm(float v)
 {
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
 }
 constexpr int kQuad0101 = 0x44;  // lanes read lanes 0, 1, 0, 1 of their quad
 constexpr int kQuad2323 = 0xEE;  // 2, 3, 2, 3
 constexpr int kQuad2301 = 0x4E;  // 2, 3, 0, 1
 
-// Bilinear lookup of one table slab at (m, c); updates (m, c). het (0 or
-// 1) selects the hom or het slab of the layer. Every lane of the wave must
-// call it together.
+// Bilinear lookup of one table slab at (m, c); updates (m, c). slab (0 or
+// kSlabBytes) selects the hom or het slab of the layer. Every lane of the
+// wave must call it together.
 //
 // Same values as gamma_smc_flow.cu:408-430 in fewer instructions: the clamp
 // is one med3 (identical for any non-NaN index), the cell corner is
@@ -159,11 +165,13 @@ constexpr int kQuad2301 = 0x4E;  // 2, 3, 0, 1
 //
 // Loads: the texture path handles a 16-byte gather four lanes at a time and
 // pays one cache access per distinct line in each such quad. Each lane needs
-// 32 contiguous bytes, so the quad's first gather fetches both halves for
-// lanes 0 and 1 and the second both halves for lanes 2 and 3 (two lines per
-// quad and gather instead of up to four), and a quad permutation hands every
-// lane its own corners.
-__device__ __forceinline__ void lookup(const char* __restrict__ layer, uint32_t het, float& m, float& c)
+// 32 contiguous bytes, its entries (r, q) and (r, q + 1), so the quad's
+// first gather reads both for lanes 0 and 1 and the second both for lanes 2
+// and 3 (two lines per quad and gather instead of up to four). A lane keeps
+// the (r, q + 1) entry it loaded itself and receives its (r, q) entry from
+// its partner lane (lane ^ 2) through a quad permutation that folds into the
+// multiply by (1 - wc).
+__device__ __forceinline__ void lookup(const char* __restrict__ layer, uint32_t slab, float& m, float& c)
 {
     float fm = (m - -5.0f) * kMeanStepInv;
     float fc = (c - -2.0f) * kCvStepInv;
@@ -173,25 +181,23 @@ __device__ __forceinline__ void lookup(const char* __restrict__ layer, uint32_t 
     const float lc = fminf(floorf(fc), 48.0f);
     const float wm = fm - lm;
     const float wc = fc - lc;
-    const uint32_t off = __umul24(het, kSlabBytes) + (uint32_t)fmaf(lm, 800.0f, lc * 16.0f);
+    // entry index r * 50 + q is an exact small integer in float
+    const uint32_t off = ((uint32_t)fmaf(lm, 50.0f, lc) << 4) + slab;
     const bool upper = (__lane_id() & 2u) != 0;
-    // first gather: lanes 0, 1 their entry (r, q); lanes 2, 3 entry (r, q + 1) of lanes 0, 1
-    const uint32_t a1 = quad_perm<kQuad0101>(off) + (upper ? 16u : 0u);
-    // second gather: lanes 2, 3 their entry (r, q); lanes 0, 1 entry (r, q + 1) of lanes 2, 3
-    const uint32_t a2 = quad_perm<kQuad2323>(off) + (upper ? 0u : 16u);
+    // first gather: lanes 0, 1 their entry (r, q + 1); lanes 2, 3 entry (r, q) of lanes 0, 1
+    const uint32_t a1 = quad_perm<kQuad0101>(off) + (upper ? 0u : 16u);
+    // second gather: lanes 2, 3 their entry (r, q + 1); lanes 0, 1 entry (r, q) of lanes 2, 3
+    const uint32_t a2 = quad_perm<kQuad2323>(off) + (upper ? 16u : 0u);
     const float4 x = *reinterpret_cast<const float4*>(layer + a1);
     const float4 y = *reinterpret_cast<const float4*>(layer + a2);
-    // own entry (r, q) = {v00, v10}; own entry (r, q + 1) = {v01, v11}
-    const float4 e0 = upper ? y : x;
+    // own entry (r, q + 1) = {v01, v11}; the partner's copy of own entry (r, q) = {v00, v10}
+    const float4 e1 = upper ? y : x;
     const float4 z = upper ? x : y;
-    float4 e1;
-    e1.x = quad_perm<kQuad2301>(z.x);
-    e1.y = quad_perm<kQuad2301>(z.y);
-    e1.z = quad_perm<kQuad2301>(z.z);
-    e1.w = quad_perm<kQuad2301>(z.w);
     const float omwm = 1.0f - wm, omwc = 1.0f - wc;
-    m = fmaf(wm, fmaf(wc, e1.z, omwc * e0.z), omwm * fmaf(wc, e1.x, omwc * e0.x));
-    c = fmaf(wm, fmaf(wc, e1.w, omwc * e0.w), omwm * fmaf(wc, e1.y, omwc * e0.y));
+    const float p00m = omwc * quad_perm<kQuad2301>(z.x), p10m = omwc * quad_perm<kQuad2301>(z.z);
+    const float p00c = omwc * quad_perm<kQuad2301>(z.y), p10c = omwc * quad_perm<kQuad2301>(z.w);
+    m = fmaf(wm, fmaf(wc, e1.z, p10m), omwm * fmaf(wc, e1.x, p00m));
+    c = fmaf(wm, fmaf(wc, e1.w, p10c), omwm * fmaf(wc, e1.y, p00c));
 }
 
 __device__ __forceinline__ const char* layer_of(const char* base, uint32_t d)
@@ -201,17 +207,17 @@ __device__ __forceinline__ const char* layer_of(const char* base, uint32_t d)
 
 // Forward segment: full N-bp hom stretches, then the final table of
 // (remaining bp, observed site).
-__device__ __forceinline__ void fwd_segment(const DevTables& t, uint32_t d, uint32_t het, float& m, float& c)
+__device__ __forceinline__ void fwd_segment(const DevTables& t, uint32_t d, uint32_t slab, float& m, float& c)
 {
     for (uint32_t q = d >> kDescLayerBits; q != 0; --q) lookup(t.stretch, 0, m, c);
-    lookup(layer_of(t.fwd, d), het, m, c);
+    lookup(layer_of(t.fwd, d), slab, m, c);
 }
 
 // Backward segment: observed site with the remaining bp first, then the
 // full N-bp stretches.
-__device__ __forceinline__ void bwd_segment(const DevTables& t, uint32_t d, uint32_t het, float& m, float& c)
+__device__ __forceinline__ void bwd_segment(const DevTables& t, uint32_t d, uint32_t slab, float& m, float& c)
 {
-    lookup(layer_of(t.bwd, d), het, m, c);
+    lookup(layer_of(t.bwd, d), slab, m, c);
     for (uint32_t q = d >> kDescLayerBits; q != 0; --q) lookup(t.stretch, 0, m, c);
 }
 
@@ -240,6 +246,18 @@ __device__ __forceinline__ uint32_t het_bit(uint64_t bits, int k)
 {
     const uint32_t word = k < 32 ? (uint32_t)bits : (uint32_t)(bits >> 32);
     return (word >> (k & 31)) & 1u;
+}
+
+// Byte offset of the slab site k of a het mask selects: 0 (hom) or
+// kSlabBytes (het): a sign-extending bit extract (0 or all ones) and a mask,
+// on 32 bits. The extract is written out because the compiler otherwise
+// rewrites the pair into a 64-bit compare and a select (four instructions).
+__device__ __forceinline__ uint32_t het_slab(uint64_t bits, int k)
+{
+    const uint32_t word = k < 32 ? (uint32_t)bits : (uint32_t)(bits >> 32);
+    uint32_t ext;
+    asm("v_bfe_i32 %0, %1, %2, 1" : "=v"(ext) : "v"(word), "n"(k & 31));
+    return ext & kSlabBytes;
 }
 
 // K consecutive site descriptors from s (the array is padded by 64 entries
@@ -310,8 +328,8 @@ template <int NP> struct Lanes {
     const uint64_t* rj[NP];
     uint32_t col[NP];   // byte offset of the pair in an output row
     bool live[NP];      // the pair exists (idle lanes shadow the job's last pair)
-    uint32_t ck_col;    // byte offset of the lane in a checkpoint row of 64 NP float2
-    float2* ck;         // checkpoint rows of the item: window w at ck + w * 64 NP
+    uint32_t ck_col;    // byte offset of the lane in a state row of 64 NP float2
+    float2* ck;         // state rows of the item: row w at ck + w * 64 NP
     int32_t grp0;       // 64-pair group of slot q = 0 (groups are tile * NP + q)
 };
 
@@ -332,6 +350,21 @@ template <int NP>
 __device__ __forceinline__ float2* ck_at(const Lanes<NP>& L, int64_t w, int q)
 {
     return global_at<float2>(L.ck + (size_t)w * 64 * NP, L.ck_col + (uint32_t)q * 512u);
+}
+
+// Stored forward states go out and come back once each, a kernel apart:
+// streaming accesses that leave the caches to the transition tables.
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void put_state(float2* p, float m, float c)
+{
+    f32x2 v;
+    v.x = m;
+    v.y = c;
+    __builtin_nontemporal_store(v, reinterpret_cast<f32x2*>(p));
+}
+__device__ __forceinline__ f32x2 get_state(const float2* p)
+{
+    return __builtin_nontemporal_load(reinterpret_cast<const f32x2*>(p));
 }
 
 // Per-site reduction of one window's means over the 64 lanes of each pair
@@ -452,9 +485,9 @@ __device__ __forceinline__ void store_codes(const FbArgs& a, const JobDev& j, co
 // store (vector-memory counters are shared by loads and stores). FAST (a
 // full window and a full next window, no long gaps in either) has no branch
 // inside the loop, so the compiler can interleave all chains.
-template <int K, int NP, bool CI, bool POST, bool SUM, bool B, bool FAST>
-__device__ __forceinline__ void window_step(const FbArgs& a, const JobDev& j, const Lanes<NP>& L, int64_t ws,
-                                            int64_t cnt, bool has_next, const Window<K, NP>& cur,
+template <int K, int NP, bool CI, bool POST, bool SUM, bool B, bool FAST, bool ST>
+__device__ __forceinline__ void window_step(const FbArgs& a, const JobDev& j, const Lanes<NP>& L, int64_t cs,
+                                            int64_t ws, int64_t cnt, bool has_next, const Window<K, NP>& cur,
                                             const Window<K, NP>& nxt, float (&m)[NP], float (&c)[NP],
                                             float (&fm)[NP][K], float (&fc)[NP][K], float (&ym)[NP],
                                             float (&yc)[NP], float* red)
@@ -469,9 +502,9 @@ __device__ __forceinline__ void window_step(const FbArgs& a, const JobDev& j, co
             for (int q = 0; q < NP; ++q) {
                 out[q][k] = combine_mean(fm[q][slot], fc[q][slot], ym[q], yc[q], a.unscale, oa[q][k], ob[q][k]);
                 if (FAST)
-                    lookup(layer_of(a.t.bwd, cur.d[k]), het_bit(cur.bits[q], k), ym[q], yc[q]);
+                    lookup(layer_of(a.t.bwd, cur.d[k]), het_slab(cur.bits[q], k), ym[q], yc[q]);
                 else
-                    bwd_segment(a.t, cur.d[k], het_bit(cur.bits[q], k), ym[q], yc[q]);
+                    bwd_segment(a.t, cur.d[k], het_slab(cur.bits[q], k), ym[q], yc[q]);
             }
         } else if (SUM) {
 #pragma unroll
@@ -480,10 +513,17 @@ __device__ __forceinline__ void window_step(const FbArgs& a, const JobDev& j, co
         if (FAST || has_next) {
 #pragma unroll
             for (int q = 0; q < NP; ++q) {
+                if (ST) {
+                    // next window: site ws - K + i, read back into the slot just freed
+                    const f32x2 f = get_state(ck_at(L, ws - K + i - cs, q));
+                    fm[q][slot] = f.x;
+                    fc[q][slot] = f.y;
+                    continue;
+                }
                 if (FAST)
-                    lookup(layer_of(a.t.fwd, nxt.d[i]), het_bit(nxt.bits[q], i), m[q], c[q]);
+                    lookup(layer_of(a.t.fwd, nxt.d[i]), het_slab(nxt.bits[q], i), m[q], c[q]);
                 else
-                    fwd_segment(a.t, nxt.d[i], het_bit(nxt.bits[q], i), m[q], c[q]);
+                    fwd_segment(a.t, nxt.d[i], het_slab(nxt.bits[q], i), m[q], c[q]);
                 fm[q][slot] = m[q];  // next window: site ws - K + i, in the slot just freed
                 fc[q][slot] = c[q];
             }
@@ -550,8 +590,8 @@ __device__ __forceinline__ ItemView<NP> open_item(const FbArgs& a, int item_in, 
         v.L.ri[q] = a.packed + (size_t)hi * (size_t)a.W;
         v.L.rj[q] = a.packed + (size_t)hj * (size_t)a.W;
     }
-    // Every lane, idle ones included, owns its checkpoint columns.
-    v.L.ck = a.ck + (size_t)v.b.ck_off + (size_t)it.tile * (size_t)v.b.nwin * 64 * NP;
+    // Every lane, idle ones included, owns its state columns.
+    v.L.ck = a.ck + it.ck_off;
     v.L.ck_col = (uint32_t)lane * 8u;
     v.L.grp0 = it.tile * NP;
     return v;
@@ -563,10 +603,10 @@ template <int NP> __device__ __forceinline__ float2* yb_at(const FbArgs& a, int 
 }
 
 // First half of an item (64 NP pairs x one block window): the forward walk
-// over [ps, ce) storing the state at the start of every core window, and
-// the backward walk over the right flank [ce, pe), whose final state is
-// the backward state entering the core.
-template <int K, int NP>
+// over [ps, ce) storing the state at the start of every core window (ST:
+// the state after every core site), and the backward walk over the right
+// flank [ce, pe), whose final state is the backward state entering the core.
+template <int K, int NP, bool ST>
 __device__ __forceinline__ void forward_item(const FbArgs& a, int item, int lane)
 {
     const ItemView<NP> v = open_item<NP>(a, item, lane);
@@ -587,7 +627,8 @@ __device__ __forceinline__ void forward_item(const FbArgs& a, int item, int lane
     }
     for (int64_t s = ps; s < ce;) {
         const int64_t e = (s < cs) ? min(cs, s + K) : min(ce, s + K);
-        if (s >= cs) {
+        const bool core = s >= cs;  // windows are all flank or all core
+        if (!ST && core) {
 #pragma unroll
             for (int q = 0; q < NP; ++q) *ck_at(L, (s - cs) / K, q) = make_float2(m[q], c[q]);
         }
@@ -598,13 +639,19 @@ __device__ __forceinline__ void forward_item(const FbArgs& a, int item, int lane
 #pragma unroll
             for (int k = 0; k < K; ++k)
 #pragma unroll
-                for (int q = 0; q < NP; ++q) lookup(layer_of(a.t.fwd, win.d[k]), het_bit(win.bits[q], k), m[q], c[q]);
+                for (int q = 0; q < NP; ++q) {
+                    lookup(layer_of(a.t.fwd, win.d[k]), het_slab(win.bits[q], k), m[q], c[q]);
+                    if (ST && core) put_state(ck_at(L, s - cs + k, q), m[q], c[q]);
+                }
         } else {
 #pragma unroll
             for (int k = 0; k < K; ++k)
                 if (s + k < e)
 #pragma unroll
-                    for (int q = 0; q < NP; ++q) fwd_segment(a.t, win.d[k], het_bit(win.bits[q], k), m[q], c[q]);
+                    for (int q = 0; q < NP; ++q) {
+                        fwd_segment(a.t, win.d[k], het_slab(win.bits[q], k), m[q], c[q]);
+                        if (ST && core) put_state(ck_at(L, s - cs + k, q), m[q], c[q]);
+                    }
         }
         s = e;
     }
@@ -620,13 +667,13 @@ __device__ __forceinline__ void forward_item(const FbArgs& a, int item, int lane
 #pragma unroll
             for (int k = K - 1; k >= 0; --k)
 #pragma unroll
-                for (int q = 0; q < NP; ++q) lookup(layer_of(a.t.bwd, win.d[k]), het_bit(win.bits[q], k), ym[q], yc[q]);
+                for (int q = 0; q < NP; ++q) lookup(layer_of(a.t.bwd, win.d[k]), het_slab(win.bits[q], k), ym[q], yc[q]);
         } else {
 #pragma unroll
             for (int k = K - 1; k >= 0; --k)
                 if (b0 + k < e)
 #pragma unroll
-                    for (int q = 0; q < NP; ++q) bwd_segment(a.t, win.d[k], het_bit(win.bits[q], k), ym[q], yc[q]);
+                    for (int q = 0; q < NP; ++q) bwd_segment(a.t, win.d[k], het_slab(win.bits[q], k), ym[q], yc[q]);
         }
         e = b0;
     }
@@ -636,8 +683,10 @@ __device__ __forceinline__ void forward_item(const FbArgs& a, int item, int lane
 
 // Second half of an item: the core windows, last to first. The last window
 // (possibly partial) is recomputed alone into orientation A; every later
-// recompute rides along the previous window's backward walk.
-template <int K, int NP, bool CI, bool POST, bool SUM>
+// recompute rides along the previous window's backward walk. ST: the
+// forward states are read back instead of recomputed, so the next
+// window's descriptors and het bits are not needed.
+template <int K, int NP, bool CI, bool POST, bool SUM, bool ST>
 __device__ __forceinline__ void backward_item(const FbArgs& a, int item, int lane, float* red)
 {
     const ItemView<NP> v = open_item<NP>(a, item, lane);
@@ -653,29 +702,44 @@ __device__ __forceinline__ void backward_item(const FbArgs& a, int item, int lan
         const float2 y = *yb_at<NP>(a, __builtin_amdgcn_readfirstlane(item), lane, q);
         ym[q] = y.x;
         yc[q] = y.y;
+        m[q] = c[q] = 0.0f;
     }
     float fm[NP][K], fc[NP][K];
     int64_t w = (int64_t)b.nwin - 1;
     int64_t cnt = ce - (cs + w * K);
     {
         const int64_t ws = cs + w * K;
-        Window<K, NP> win;
-        load_window(a, L, ws, win);
-        if (ws == ps) win.d[0] = b.sd_fwd;
+        if (ST) {
 #pragma unroll
-        for (int q = 0; q < NP; ++q) {
-            const float2 st = *ck_at(L, w, q);
-            m[q] = st.x;
-            c[q] = st.y;
-        }
+            for (int k = 0; k < K; ++k) {
+                if (k < cnt) {
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            if (k < cnt) {
+                    for (int q = 0; q < NP; ++q) {
+                        const f32x2 f = get_state(ck_at(L, ws + k - cs, q));
+                        fm[q][k] = f.x;
+                        fc[q][k] = f.y;
+                    }
+                }
+            }
+        } else {
+            Window<K, NP> win;
+            load_window(a, L, ws, win);
+            if (ws == ps) win.d[0] = b.sd_fwd;
 #pragma unroll
-                for (int q = 0; q < NP; ++q) {
-                    fwd_segment(a.t, win.d[k], het_bit(win.bits[q], k), m[q], c[q]);
-                    fm[q][k] = m[q];
-                    fc[q][k] = c[q];
+            for (int q = 0; q < NP; ++q) {
+                const float2 st = *ck_at(L, w, q);
+                m[q] = st.x;
+                c[q] = st.y;
+            }
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                if (k < cnt) {
+#pragma unroll
+                    for (int q = 0; q < NP; ++q) {
+                        fwd_segment(a.t, win.d[k], het_slab(win.bits[q], k), m[q], c[q]);
+                        fm[q][k] = m[q];
+                        fc[q][k] = c[q];
+                    }
                 }
             }
         }
@@ -687,7 +751,7 @@ __device__ __forceinline__ void backward_item(const FbArgs& a, int item, int lan
         Window<K, NP> cur, nxt;
         load_window(a, L, ws, cur);
         if (ws == ps) cur.d[0] = b.sd_bwd;
-        if (has_next) {
+        if (has_next && !ST) {
             load_window(a, L, ws - K, nxt);
             if (ws - K == ps) nxt.d[0] = b.sd_fwd;
 #pragma unroll
@@ -702,17 +766,21 @@ __device__ __forceinline__ void backward_item(const FbArgs& a, int item, int lan
 #pragma unroll
             for (int q = 0; q < NP; ++q) nxt.bits[q] = 0;
         }
-        const bool fast = has_next && cnt == K && no_long_gaps<K>(cur.d) && no_long_gaps<K>(nxt.d);
+        const bool fast = has_next && cnt == K && no_long_gaps<K>(cur.d) && (ST || no_long_gaps<K>(nxt.d));
         if (fast) {
             if (orient_b)
-                window_step<K, NP, CI, POST, SUM, true, true>(a, j, L, ws, cnt, true, cur, nxt, m, c, fm, fc, ym, yc, red);
+                window_step<K, NP, CI, POST, SUM, true, true, ST>(a, j, L, cs, ws, cnt, true, cur, nxt, m, c, fm, fc, ym,
+                                                                  yc, red);
             else
-                window_step<K, NP, CI, POST, SUM, false, true>(a, j, L, ws, cnt, true, cur, nxt, m, c, fm, fc, ym, yc, red);
+                window_step<K, NP, CI, POST, SUM, false, true, ST>(a, j, L, cs, ws, cnt, true, cur, nxt, m, c, fm, fc, ym,
+                                                                   yc, red);
         } else {
             if (orient_b)
-                window_step<K, NP, CI, POST, SUM, true, false>(a, j, L, ws, cnt, has_next, cur, nxt, m, c, fm, fc, ym, yc, red);
+                window_step<K, NP, CI, POST, SUM, true, false, ST>(a, j, L, cs, ws, cnt, has_next, cur, nxt, m, c, fm, fc,
+                                                                   ym, yc, red);
             else
-                window_step<K, NP, CI, POST, SUM, false, false>(a, j, L, ws, cnt, has_next, cur, nxt, m, c, fm, fc, ym, yc, red);
+                window_step<K, NP, CI, POST, SUM, false, false, ST>(a, j, L, cs, ws, cnt, has_next, cur, nxt, m, c, fm,
+                                                                    fc, ym, yc, red);
         }
         if (!has_next) break;
         orient_b = !orient_b;
@@ -722,8 +790,9 @@ __device__ __forceinline__ void backward_item(const FbArgs& a, int item, int lan
 }
 
 // Waves per SIMD the backward kernel is built for: three with one pair per
-// lane (a few registers over would leave two), no floor with two.
-template <int NP> constexpr int kBwdWaves = NP == 1 ? 3 : 1;
+// lane and recomputed forward states (a few registers over would leave
+// two), no floor otherwise.
+template <int NP, bool ST> constexpr int kBwdWaves = (NP == 1 && !ST) ? 3 : 1;
 
 // A decode is two launches over the same work items: fb_fwd_* (forward
 // walk + right-flank backward walk; few registers, so many waves per SIMD
@@ -731,19 +800,19 @@ template <int NP> constexpr int kBwdWaves = NP == 1 ? 3 : 1;
 // pipelined core windows). Static schedule: workgroup g decodes item g;
 // persistent schedule: waves take items from a device counter until it
 // passes n_items, so every wave exits.
-template <int K, int NP>
+template <int K, int NP, bool ST>
 __global__ __launch_bounds__(64) void fb_fwd_items(FbArgs a)
 {
     const int item = __builtin_amdgcn_readfirstlane((int)blockIdx.x);
-    if (item < a.n_items) forward_item<K, NP>(a, item, threadIdx.x);
+    if (item < a.n_items) forward_item<K, NP, ST>(a, item, threadIdx.x);
 }
 
-template <int K, int NP, bool CI, bool POST, bool SUM>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kBwdWaves<NP>))) void fb_bwd_items(FbArgs a)
+template <int K, int NP, bool CI, bool POST, bool SUM, bool ST>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kBwdWaves<NP, ST>))) void fb_bwd_items(FbArgs a)
 {
     const int item = __builtin_amdgcn_readfirstlane((int)blockIdx.x);
     __shared__ float red[SUM ? 16 * 65 : 1];
-    if (item < a.n_items) backward_item<K, NP, CI, POST, SUM>(a, item, threadIdx.x, red);
+    if (item < a.n_items) backward_item<K, NP, CI, POST, SUM, ST>(a, item, threadIdx.x, red);
 }
 
 __device__ __forceinline__ int next_item(int* counter, int lane)
@@ -753,19 +822,19 @@ __device__ __forceinline__ int next_item(int* counter, int lane)
     return __builtin_amdgcn_readfirstlane(__shfl(item, 0));
 }
 
-template <int K, int NP>
+template <int K, int NP, bool ST>
 __global__ __launch_bounds__(64) void fb_fwd_queue(FbArgs a)
 {
     for (int item = next_item(a.queue, threadIdx.x); item < a.n_items; item = next_item(a.queue, threadIdx.x))
-        forward_item<K, NP>(a, item, threadIdx.x);
+        forward_item<K, NP, ST>(a, item, threadIdx.x);
 }
 
-template <int K, int NP, bool CI, bool POST, bool SUM>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kBwdWaves<NP>))) void fb_bwd_queue(FbArgs a)
+template <int K, int NP, bool CI, bool POST, bool SUM, bool ST>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kBwdWaves<NP, ST>))) void fb_bwd_queue(FbArgs a)
 {
     __shared__ float red[SUM ? 16 * 65 : 1];
     for (int item = next_item(a.queue2, threadIdx.x); item < a.n_items; item = next_item(a.queue2, threadIdx.x))
-        backward_item<K, NP, CI, POST, SUM>(a, item, threadIdx.x, red);
+        backward_item<K, NP, CI, POST, SUM, ST>(a, item, threadIdx.x, red);
 }
 
 // Summary epilogue: per site r of a job, the groups' partials in group

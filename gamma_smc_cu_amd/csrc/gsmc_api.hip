@@ -271,11 +271,12 @@ uint32_t descriptor(int steps, int N)
 using namespace gsmc;
 
 struct Tuning {
-    int K = 16;              // checkpoint interval = register window
+    int K = 16;              // checkpoint interval = register window; 1: every forward state stored
     int np = 1;              // pairs per lane (mean-only kernel; 1 or 2)
     int schedule = 0;        // 0 static, 1 persistent queue
     int xcd_order = 0;       // static schedule: give each XCD a contiguous run of items
     int64_t min_waves = 8192;
+    uint64_t max_state_bytes = 0;  // cap on a launch's checkpoint / stored-state buffer (0: free memory)
 };
 
 struct gsmc_batch;
@@ -464,20 +465,47 @@ struct SumJob {
 };
 
 // Launch metadata: [jobs][blocks][items], 256-byte aligned sections.
+// A run of items launched as one fwd / bwd kernel pair. Phases run one
+// after the other and reuse the state buffer.
+struct Phase {
+    int64_t first = 0, count = 0;
+};
+
 struct Plan {
     std::vector<char> bytes;
     size_t off_blocks = 0, off_items = 0;
     int64_t n_items = 0;
     int64_t n_waves = 0;  // waves with work (items without padding)
-    size_t ck_elems = 0;
+    size_t ck_elems = 0;  // state buffer (float2) of the largest phase
     size_t part_elems = 0;        // summary partials (per plane)
     std::vector<SumJob> sums;
+    std::vector<Phase> phases;
 };
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// Forward states stored at every core site (instead of checkpoints every K
+// sites and a recompute): tuning K = 1, for the mean / code / summary
+// kernels. Those kernels walk windows of 16 sites.
+bool stored_states(const Tuning& tu, int flags) { return tu.K == 1 && !(flags & (GSMC_CI | GSMC_POSTERIOR)); }
+int window_k(const Tuning& tu) { return tu.K == 8 ? 8 : 16; }
+
+// Bytes the state buffer of workspace b may take for a plan needing `want`:
+// all of it when that fits in the current allocation or in the allocation
+// plus free device memory (less a reserve), else that much.
+size_t state_budget(const gsmc_context* ctx, size_t have, size_t want)
+{
+    const size_t cap = ctx->tune.max_state_bytes ? (size_t)ctx->tune.max_state_bytes : SIZE_MAX;
+    if (want <= have && want <= cap) return want;
+    size_t free_b = 0, total_b = 0;
+    HIP_OK(hipMemGetInfo(&free_b, &total_b));
+    const size_t reserve = (size_t)2 << 30;
+    const size_t avail = have + (free_b > reserve ? free_b - reserve : 0);
+    return std::min(std::min(want, avail), cap);
+}
+
 Plan make_plan(const gsmc_context* ctx, const std::vector<JobSpec>& jobs, const Tuning& tu, int np,
-               bool summary)
+               bool summary, bool stored, size_t have_ck_bytes)
 {
     std::vector<JobDev> jd;
     std::vector<BlockDev> bd;
@@ -528,11 +556,13 @@ Plan make_plan(const gsmc_context* ctx, const std::vector<JobSpec>& jobs, const 
                 d.sd_bwd = 0;
             }
             d.job = (int32_t)(jd.size() - 1);
-            d.nwin = (int32_t)((b.ce - b.cs + tu.K - 1) / tu.K);
-            d.ck_off = (int64_t)ck;
-            ck += (size_t)tiles * d.nwin * (size_t)tile_pairs;
+            const int kw = stored ? 16 : window_k(tu);
+            d.nwin = (int32_t)((b.ce - b.cs + kw - 1) / kw);
+            const int64_t rows = stored ? (int64_t)(b.ce - b.cs) : (int64_t)d.nwin;
             require(bd.size() < (size_t)INT32_MAX, "too many block windows");
-            for (int64_t t = 0; t < tiles; ++t) items.push_back({(int32_t)bd.size(), (int32_t)t});
+            // ck_off holds the item's state elements until phases place them
+            for (int64_t t = 0; t < tiles; ++t) items.push_back({(int32_t)bd.size(), (int32_t)t, rows * tile_pairs});
+            ck += (size_t)(tiles * rows * tile_pairs);
             bd.push_back(d);
         }
     }
@@ -549,25 +579,58 @@ Plan make_plan(const gsmc_context* ctx, const std::vector<JobSpec>& jobs, const 
         };
         return cost(x) > cost(y);
     });
-    if (tu.schedule == 0 && tu.xcd_order && !items.empty()) {
-        // Workgroup g runs on XCD g % 8: hand each XCD a contiguous run of
-        // workgroup-sized chunks, so the waves sharing an XCD's L2 walk the
-        // same blocks (and so the same table layers) at the same time.
-        const int64_t wpw = 1;  // 64-lane workgroups: one item each
-        const int64_t G = ((int64_t)items.size() + wpw - 1) / wpw;
-        const int64_t per = (G + 7) / 8;
-        std::vector<ItemDev> perm((size_t)(per * 8 * wpw), ItemDev{-1, 0});
-        for (int64_t g = 0; g < per * 8; ++g) {
-            const int64_t chunk = (g % 8) * per + g / 8;
-            for (int64_t w = 0; w < wpw; ++w) {
-                const int64_t src = chunk * wpw + w;
-                if (chunk < G && src < (int64_t)items.size()) perm[(size_t)(g * wpw + w)] = items[(size_t)src];
-            }
-        }
-        items.swap(perm);
+    // Phases: when the states of all items exceed the budget, deal the
+    // items round-robin (longest first) into the fewest phases that fit, so
+    // every phase gets the same mix of long and short items.
+    const size_t budget = state_budget(ctx, have_ck_bytes, ck * sizeof(float2)) / sizeof(float2);
+    int64_t n_ph = 1;
+    if (ck > budget) {
+        size_t biggest = 0;
+        for (const ItemDev& it : items) biggest = std::max(biggest, (size_t)it.ck_off);
+        if (biggest > budget)
+            throw Error(GSMC_ENOMEM, "state budget below the checkpoints / forward states of one work item (" +
+                                         std::to_string(biggest * sizeof(float2)) + " bytes)");
+        n_ph = (int64_t)((ck + budget - 1) / budget);
     }
+    std::vector<std::vector<ItemDev>> ph;
+    for (;; ++n_ph) {
+        ph.assign((size_t)n_ph, {});
+        std::vector<size_t> used((size_t)n_ph, 0);
+        for (size_t i = 0; i < items.size(); ++i) {
+            ph[i % (size_t)n_ph].push_back(items[i]);
+            used[i % (size_t)n_ph] += (size_t)items[i].ck_off;
+        }
+        if (*std::max_element(used.begin(), used.end()) <= budget) break;
+    }
+    std::vector<ItemDev> all;
+    size_t ck_max = 0;
+    for (std::vector<ItemDev>& v : ph) {
+        if (tu.schedule == 0 && tu.xcd_order && !v.empty()) {
+            // Workgroup g runs on XCD g % 8: hand each XCD a contiguous run of
+            // items, so the waves sharing an XCD's L2 walk the same blocks
+            // (and so the same table layers) at the same time.
+            const int64_t G = (int64_t)v.size();
+            const int64_t per = (G + 7) / 8;
+            std::vector<ItemDev> perm((size_t)(per * 8), ItemDev{-1, 0, 0});
+            for (int64_t g = 0; g < per * 8; ++g) {
+                const int64_t src = (g % 8) * per + g / 8;
+                if (src < G) perm[(size_t)g] = v[(size_t)src];
+            }
+            v.swap(perm);
+        }
+        size_t off = 0;
+        for (ItemDev& it : v) {
+            const size_t n = it.block >= 0 ? (size_t)it.ck_off : 0;
+            it.ck_off = (int64_t)off;
+            off += n;
+        }
+        ck_max = std::max(ck_max, off);
+        pl.phases.push_back({(int64_t)all.size(), (int64_t)v.size()});
+        all.insert(all.end(), v.begin(), v.end());
+    }
+    items.swap(all);
     pl.n_items = (int64_t)items.size();
-    pl.ck_elems = ck;
+    pl.ck_elems = ck_max;
     pl.part_elems = part;
     pl.sums = std::move(sums);
     const size_t sj = jd.size() * sizeof(JobDev), sb = bd.size() * sizeof(BlockDev);
@@ -582,7 +645,10 @@ Plan make_plan(const gsmc_context* ctx, const std::vector<JobSpec>& jobs, const 
 
 // Pairs per lane a launch uses: the mean-only kernels (the throughput path)
 // are built for 1 and 2, the CI / posterior kernels for 1.
-int launch_np(const Tuning& tu, int flags) { return (flags & (GSMC_CI | GSMC_POSTERIOR)) ? 1 : tu.np; }
+int launch_np(const Tuning& tu, int flags)
+{
+    return ((flags & (GSMC_CI | GSMC_POSTERIOR)) || tu.K == 1) ? 1 : tu.np;
+}
 
 // Kernel pair of one launch shape, and each kernel's resident waves.
 struct KernelPair {
@@ -592,12 +658,12 @@ struct KernelPair {
     const void* bwd_queue;
 };
 
-template <int K, int NP, bool C, bool Q, bool SUM = false> KernelPair kernels_of()
+template <int K, int NP, bool C, bool Q, bool SUM = false, bool ST = false> KernelPair kernels_of()
 {
-    return {reinterpret_cast<const void*>(&fb_fwd_items<K, NP>),
-            reinterpret_cast<const void*>(&fb_bwd_items<K, NP, C, Q, SUM>),
-            reinterpret_cast<const void*>(&fb_fwd_queue<K, NP>),
-            reinterpret_cast<const void*>(&fb_bwd_queue<K, NP, C, Q, SUM>)};
+    return {reinterpret_cast<const void*>(&fb_fwd_items<K, NP, ST>),
+            reinterpret_cast<const void*>(&fb_bwd_items<K, NP, C, Q, SUM, ST>),
+            reinterpret_cast<const void*>(&fb_fwd_queue<K, NP, ST>),
+            reinterpret_cast<const void*>(&fb_bwd_queue<K, NP, C, Q, SUM, ST>)};
 }
 
 template <int K> KernelPair kernels_k(const Tuning& tu, int flags)
@@ -613,7 +679,10 @@ template <int K> KernelPair kernels_k(const Tuning& tu, int flags)
 
 KernelPair kernels_for(const Tuning& tu, int flags)
 {
-    return tu.K == 8 ? kernels_k<8>(tu, flags) : kernels_k<16>(tu, flags);
+    if (stored_states(tu, flags))
+        return (flags & GSMC_SUMMARY) ? kernels_of<16, 1, false, false, true, true>()
+                                      : kernels_of<16, 1, false, false, false, true>();
+    return window_k(tu) == 8 ? kernels_k<8>(tu, flags) : kernels_k<16>(tu, flags);
 }
 
 int64_t resident(const gsmc_context* ctx, const void* fn)
@@ -681,13 +750,21 @@ void enqueue_kernels(gsmc_context* ctx, const Plan& pl, const FbArgs& a, int fla
 {
     const Tuning& tu = ctx->tune;
     const KernelPair kp = kernels_for(tu, flags);
-    if (tu.schedule == 1) {
-        hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(1), 0, stream, a.queue);
-        launch(kp.fwd_queue, (unsigned)std::max<int64_t>(1, std::min(resident(ctx, kp.fwd_queue), pl.n_items)), a, stream);
-        launch(kp.bwd_queue, (unsigned)std::max<int64_t>(1, std::min(resident(ctx, kp.bwd_queue), pl.n_items)), a, stream);
-    } else {
-        launch(kp.fwd_items, (unsigned)pl.n_items, a, stream);
-        launch(kp.bwd_items, (unsigned)pl.n_items, a, stream);
+    for (const Phase& ph : pl.phases) {
+        if (ph.count == 0) continue;
+        FbArgs pa = a;
+        pa.items = a.items + ph.first;
+        pa.n_items = ph.count;
+        if (tu.schedule == 1) {
+            hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(1), 0, stream, a.queue);
+            launch(kp.fwd_queue, (unsigned)std::max<int64_t>(1, std::min(resident(ctx, kp.fwd_queue), ph.count)), pa,
+                   stream);
+            launch(kp.bwd_queue, (unsigned)std::max<int64_t>(1, std::min(resident(ctx, kp.bwd_queue), ph.count)), pa,
+                   stream);
+        } else {
+            launch(kp.fwd_items, (unsigned)ph.count, pa, stream);
+            launch(kp.bwd_items, (unsigned)ph.count, pa, stream);
+        }
     }
     for (const SumJob& sj : pl.sums) {
         const JobSpec& js = *sj.spec;
@@ -706,7 +783,7 @@ void launch_plan(gsmc_batch* b, const std::vector<JobSpec>& jobs, int flags, hip
     gsmc_context* ctx = b->ctx;
     const Tuning tu = ctx->tune;
     const bool summary = (flags & GSMC_SUMMARY) != 0;
-    Plan pl = make_plan(ctx, jobs, tu, launch_np(tu, flags), summary);
+    Plan pl = make_plan(ctx, jobs, tu, launch_np(tu, flags), summary, stored_states(tu, flags), b->ck.bytes);
     if (pl.n_items == 0) return;
     const bool same = b->pending && stream == b->last_stream && pl.bytes == b->last;
     if (!same) {
@@ -1071,7 +1148,7 @@ int gsmc_context_set_tuning(gsmc_context* ctx, int k, int pairs_per_lane, int sc
 {
     return guarded([&] {
         require(ctx != nullptr, "context is NULL");
-        require(k == 8 || k == 16, "checkpoint_k must be 8 or 16");
+        require(k == 1 || k == 8 || k == 16, "checkpoint_k must be 1 (stored forward states), 8 or 16");
         require(pairs_per_lane == 1 || pairs_per_lane == 2, "pairs_per_lane must be 1 or 2");
         require(schedule == 0 || schedule == 1, "schedule must be 0 (static) or 1 (persistent)");
         require(min_waves >= 1, "min_waves_per_launch must be positive");
@@ -1081,6 +1158,15 @@ int gsmc_context_set_tuning(gsmc_context* ctx, int k, int pairs_per_lane, int sc
         ctx->tune.schedule = schedule;
         ctx->tune.xcd_order = xcd_order ? 1 : 0;
         ctx->tune.min_waves = min_waves;
+    });
+}
+
+int gsmc_context_set_state_budget(gsmc_context* ctx, uint64_t max_state_bytes)
+{
+    return guarded([&] {
+        require(ctx != nullptr, "context is NULL");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        ctx->tune.max_state_bytes = max_state_bytes;
     });
 }
 
@@ -1400,7 +1486,8 @@ int gsmc_batch_capture(gsmc_context* ctx, const gsmc_job* jobs, int64_t n_jobs, 
         g->ctx = ctx;
         g->buf = new_batch(ctx);
         const bool summary = (flags & GSMC_SUMMARY) != 0;
-        Plan pl = make_plan(ctx, specs, ctx->tune, launch_np(ctx->tune, flags), summary);
+        Plan pl = make_plan(ctx, specs, ctx->tune, launch_np(ctx->tune, flags), summary,
+                            stored_states(ctx->tune, flags), 0);
         require(pl.n_items > 0, "nothing to capture: every job is empty");
         ensure_buffers(g->buf.get(), pl, summary);
         HIP_OK(hipMemcpy(g->buf->meta.p, pl.bytes.data(), pl.bytes.size(), hipMemcpyHostToDevice));
@@ -1523,9 +1610,9 @@ int gsmc_checkpoint_bytes(gsmc_context* ctx, int64_t n_sites, int64_t core, int6
 {
     return guarded([&] {
         require(ctx != nullptr && bytes != nullptr, "NULL argument");
-        const int64_t tile_pairs = 64 * (int64_t)ctx->tune.np;
+        const int64_t tile_pairs = 64 * (int64_t)launch_np(ctx->tune, 0);
         const int64_t tiles = (n_pairs + tile_pairs - 1) / tile_pairs;
-        uint64_t windows = 0;
+        uint64_t windows = 0;  // state rows: one per checkpoint window, or per core site with K = 1
         for (const Block& b : make_blocks(n_sites, core > 0 ? core : std::max<int64_t>(n_sites, 1), core > 0 ? flank : 0))
             windows += (uint64_t)((b.ce - b.cs + ctx->tune.K - 1) / ctx->tune.K);
         *bytes = windows * (uint64_t)tiles * (uint64_t)tile_pairs * sizeof(float2);

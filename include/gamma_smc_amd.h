@@ -128,7 +128,11 @@ void gsmc_context_destroy(gsmc_context* ctx);
 int gsmc_context_info(const gsmc_context* ctx, int64_t* n_haps, int64_t* n_sites, int* device_id, float* Ne);
 
 /* Decoder tuning; results do not depend on any of it. checkpoint_k: sites
- * per checkpoint window, 8 or 16 (default 16); pairs_per_lane: 1 or 2
+ * per checkpoint window, 8 or 16 (default 16), or 1: the forward kernel
+ * stores its state at every core site and the backward kernel reads it
+ * back instead of recomputing it (8 B per pair and core site of device
+ * memory, launches split into phases that fit; mean, code and summary
+ * launches, CI / posterior launches use 16); pairs_per_lane: 1 or 2
  * (default 1), i.e. 64 or 128 pairs per 64-lane workgroup, for the
  * mean-only kernel (CI / posterior launches use 1); schedule: 0 static grid
  * of work items, 1 persistent waves draining a device queue (default 0);
@@ -138,6 +142,12 @@ int gsmc_context_info(const gsmc_context* ctx, int64_t* n_haps, int64_t* n_sites
  * reference has fixed 256-thread launches). */
 int gsmc_context_set_tuning(gsmc_context* ctx, int checkpoint_k, int pairs_per_lane, int schedule, int xcd_order,
                             int64_t min_waves_per_launch);
+
+/* Cap on the device memory one launch may use for checkpoints / stored
+ * forward states (0, the default: free device memory less 2 GiB). Launches
+ * needing more run as consecutive phases of work items that each fit.
+ * Results do not depend on it. No reference counterpart. */
+int gsmc_context_set_state_budget(gsmc_context* ctx, uint64_t max_state_bytes);
 
 /* Full-sequence forward-backward of n_pairs pairs (pair_i[p], pair_j[p]) into
  * host arrays laid out site-major [n_sites][n_pairs]: mean always; lower and

@@ -147,6 +147,8 @@ def test_pair_index_out_of_range_raises(core, cohort_small, flow_field_path):
 SCHEDULES = [dict(checkpoint_k=k, pairs_per_lane=np_, schedule=sch, xcd_order=xcd)
              for k in (8, 16) for np_ in (1, 2)
              for sch, xcd in (("static", True), ("static", False), ("persistent", False))]
+SCHEDULES += [dict(checkpoint_k=1, pairs_per_lane=1, schedule=sch, xcd_order=xcd)
+              for sch, xcd in (("static", True), ("persistent", False))]
 
 
 def test_launch_shapes_are_bit_identical(core, cohort_small, flow_field_path):
@@ -209,6 +211,62 @@ def test_batch_matches_alone_and_oracle(core, cohort_small, flow_field_path, tu)
         ref = O.fb_blockwise_f32(sub_G, sub_pos, pairs, T, 1e4, core_sites, flank, ci=True, posterior=True)
         for k in ("mean", "lower", "upper", "posterior_alpha", "posterior_beta"):
             assert _bits_equal(a[k], ref[k]), k
+
+
+@pytest.mark.parametrize("schedule", ["static", "persistent"])
+@pytest.mark.parametrize("budget", [0, 400_000])
+def test_stored_forward_states_bit_exact(core, cohort_small, flow_field_path, schedule, budget):
+    """checkpoint_k=1: the forward kernel stores the state after every core
+    site and the backward kernel reads it back instead of recomputing it.
+    Means, log codes and ragged batches equal the oracle's bit for bit,
+    also when a state budget of about one work item (700 sites x 64 pairs
+    x 8 B = 358,400 B) splits each launch into many phases."""
+    from gamma_smc_cu_amd import batch
+
+    G, pos = cohort_small
+    pairs = O.all_pairs(G.shape[0]) + [(3, 3)]
+    T = _tables(flow_field_path, 1.25e-8, 1e-8)
+    ctx = core.FlowContext(G, pos, 1e4, 1.25e-8, 1e-8, flow_field_path, 0)
+    ctx.set_tuning(checkpoint_k=1, pairs_per_lane=1, schedule=schedule)
+    ctx.set_state_budget(budget)
+    _case(f"stored states {schedule} budget={budget}: full sequence")
+    got = ctx.run_fb(pairs, mean_only=True)
+    assert _bits_equal(got["mean"], O.fb_window_f32(G, pos, pairs, T, 1e4)["mean"])
+    for csz, fl in ((200, 64), (37, 5), (1, 3)):
+        _case(f"stored states {schedule} budget={budget}: core={csz} flank={fl}")
+        got = ctx.run_fb_blockwise(pairs, core_block_sites=csz, flank_sites=fl, pair_batch_size=-1, max_streams=1)
+        ref = O.fb_blockwise_f32(G, pos, pairs, T, 1e4, csz, fl)["mean"]
+        assert _bits_equal(got["mean"], ref), (csz, fl)
+    _case(f"stored states {schedule} budget={budget}: u8 codes")
+    ref = O.fb_blockwise_f32(G, pos, pairs, T, 1e4, 200, 64)["mean"]
+    want = O.log_codes(ref, 8, 1e4)
+    codes = ctx.run_fb_codes(pairs, bits=8, core_block_sites=200, flank_sites=64)["codes"]
+    d = np.abs(codes.astype(np.int32) - want.astype(np.int32))
+    assert d.max() <= 1 and np.count_nonzero(d) <= 1e-3 * d.size
+    _case(f"stored states {schedule} budget={budget}: ragged batch")
+    jobs = _ragged_jobs(G.shape[0], G.shape[1], np.random.default_rng(29))
+    mixed = batch.decode_batch(ctx, jobs, mean_only=True)
+    for j, a in zip(jobs, mixed):
+        pairs_j = [tuple(map(int, p)) for p in j.pair_array()]
+        if j.n_sites == 0 or not pairs_j:
+            assert a["mean"].size == 0
+            continue
+        core_sites = j.core_block_sites if j.core_block_sites > 0 else j.n_sites
+        flank = j.flank_sites if j.core_block_sites > 0 else 0
+        ref = O.fb_blockwise_f32(np.ascontiguousarray(G[:, j.site_lo:j.site_hi]),
+                                 np.ascontiguousarray(pos[j.site_lo:j.site_hi]), pairs_j, T, 1e4, core_sites, flank)
+        assert _bits_equal(a["mean"], ref["mean"])
+
+
+def test_state_budget_below_one_item_raises(core, cohort_small, flow_field_path):
+    G, pos = cohort_small
+    ctx = core.FlowContext(G, pos, 1e4, 1.25e-8, 1e-8, flow_field_path, 0)
+    ctx.set_tuning(checkpoint_k=1)
+    ctx.set_state_budget(1000)
+    with pytest.raises(MemoryError, match="one work item"):
+        ctx.run_fb([(1, 0)], mean_only=True)
+    ctx.set_state_budget(0)
+    assert ctx.run_fb([(1, 0)], mean_only=True)["mean"].shape == (G.shape[1], 1)
 
 
 def test_device_path_matches_host_path(core, cohort_small, flow_field_path):
