@@ -338,12 +338,50 @@ template <int K, int NP> struct Window {
     uint64_t bits[NP]; // het masks
 };
 
+// The het words X[w] = row_i[w] ^ row_j[w] around a walk's current window.
+// A window's 64-site mask needs X[w] and X[w + 1] (w = first site / 64);
+// windows of one walk step through the sites in order, so neighbouring
+// windows share those words and the walk loads each word once (each load
+// is a gather over 64 rows) instead of once per window.
+template <int NP> struct HetCache {
+    int64_t w = -4;           // word index of lo (uniform over the wave)
+    uint64_t lo[NP], hi[NP];  // X[w], X[w + 1]; 0 past the end of the rows
+};
+
+template <int NP> __device__ __forceinline__ uint64_t het_word(const Lanes<NP>& L, int q, int64_t w, int64_t W)
+{
+    return (w < W) ? (L.ri[q][w] ^ L.rj[q][w]) : 0ull;
+}
+
 template <int K, int NP>
-__device__ __forceinline__ void load_window(const FbArgs& a, const Lanes<NP>& L, int64_t ws, Window<K, NP>& win)
+__device__ __forceinline__ void load_window(const FbArgs& a, const Lanes<NP>& L, int64_t ws, Window<K, NP>& win,
+                                            HetCache<NP>& h)
 {
     load_descs<K>(a.desc, ws, win.d);
+    const int64_t w = ws >> 6;
+    if (w == h.w + 1) {  // walking forward: slide up one word
 #pragma unroll
-    for (int q = 0; q < NP; ++q) win.bits[q] = het_mask(L.ri[q], L.rj[q], ws, a.W);
+        for (int q = 0; q < NP; ++q) {
+            h.lo[q] = h.hi[q];
+            h.hi[q] = het_word(L, q, w + 1, a.W);
+        }
+    } else if (w == h.w - 1) {  // walking backward: slide down one word
+#pragma unroll
+        for (int q = 0; q < NP; ++q) {
+            h.hi[q] = h.lo[q];
+            h.lo[q] = het_word(L, q, w, a.W);
+        }
+    } else if (w != h.w) {
+#pragma unroll
+        for (int q = 0; q < NP; ++q) {
+            h.lo[q] = het_word(L, q, w, a.W);
+            h.hi[q] = het_word(L, q, w + 1, a.W);
+        }
+    }
+    h.w = w;
+    const int sh = (int)(ws & 63);
+#pragma unroll
+    for (int q = 0; q < NP; ++q) win.bits[q] = sh == 0 ? h.lo[q] : (h.lo[q] >> sh) | (h.hi[q] << (64 - sh));
 }
 
 template <int NP>
@@ -352,19 +390,62 @@ __device__ __forceinline__ float2* ck_at(const Lanes<NP>& L, int64_t w, int q)
     return global_at<float2>(L.ck + (size_t)w * 64 * NP, L.ck_col + (uint32_t)q * 512u);
 }
 
-// Stored forward states go out and come back once each, a kernel apart:
-// streaming accesses that leave the caches to the transition tables.
 typedef float f32x2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ void put_state(float2* p, float m, float c)
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// Stored forward states: one 16-byte record per lane and pair of core
+// sites (cs + 2t, cs + 2t + 1), records t of the item's 64 NP lanes in row
+// t. The address path spends the same cycles on a 16-byte access as on an
+// 8-byte one, so pairing sites halves its share of the states. The records
+// go out and come back once each, a kernel apart: streaming accesses that
+// leave the caches to the transition tables.
+template <int NP> __device__ __forceinline__ f32x4* rec_at(const Lanes<NP>& L, int64_t t, int q)
 {
-    f32x2 v;
-    v.x = m;
-    v.y = c;
-    __builtin_nontemporal_store(v, reinterpret_cast<f32x2*>(p));
+    return global_at<f32x4>(L.ck + (size_t)t * 128 * NP, L.ck_col * 2u + (uint32_t)q * 1024u);
 }
-__device__ __forceinline__ f32x2 get_state(const float2* p)
+__device__ __forceinline__ void put_rec(f32x4* p, float m0, float c0, float m1, float c1)
 {
-    return __builtin_nontemporal_load(reinterpret_cast<const f32x2*>(p));
+    f32x4 v;
+    v.x = m0;
+    v.y = c0;
+    v.z = m1;
+    v.w = c1;
+    __builtin_nontemporal_store(v, p);
+}
+__device__ __forceinline__ f32x4 get_rec(const f32x4* p) { return __builtin_nontemporal_load(p); }
+
+// Float means of one full window, slot q, stored as row segments: each
+// lane writes its K means down one column of an LDS tile [K][64], then
+// each lane stores V consecutive pairs of one row (V = 2 or 4: 8- or
+// 16-byte stores), K / V store instructions instead of K. The address
+// path handles a store one 4-lane quad per cycle whatever its width, so
+// wider stores cost it proportionally fewer cycles. Needs the whole slot
+// live and rows aligned to V floats.
+template <int K, int NP, int V>
+__device__ __forceinline__ void store_means_rows(const JobDev& j, const Lanes<NP>& L, int q, int64_t ws,
+                                                 const float (&col)[K], float* stage)
+{
+    const uint32_t lane = __lane_id();
+#pragma unroll
+    for (int k = 0; k < K; ++k) stage[k * 64 + lane] = col[k];
+    __syncthreads();
+    constexpr uint32_t per_row = 64 / V;  // lanes per 64-pair row segment
+    const uint32_t r = lane / per_row;    // first row of this lane (the wave covers V rows per store)
+    const uint32_t c = (lane % per_row) * V;
+    // byte offset of the lane's first pair: the slot's first pair, plus c
+    const uint64_t lane_off = (uint64_t)(L.col[q] - lane * 4u + c * 4u) + (uint64_t)r * (uint64_t)j.P * 4u;
+    global_char* base = (global_char*)j.mean + (uint64_t)(ws - j.row0) * (uint64_t)j.P * 4u + lane_off;
+    const uint64_t stride = (uint64_t)V * (uint64_t)j.P * 4u;  // V rows
+#pragma unroll
+    for (int i = 0; i < K / V; ++i) {
+        const float* src = stage + (r + i * V) * 64 + c;
+        global_char* dst = base + (uint64_t)i * stride;
+        if (V == 4)
+            __builtin_nontemporal_store(*reinterpret_cast<const f32x4*>(src), (f32x4*)dst);
+        else
+            __builtin_nontemporal_store(*reinterpret_cast<const f32x2*>(src), (f32x2*)dst);
+    }
+    __syncthreads();  // the tile is rewritten by the next window
 }
 
 // Per-site reduction of one window's means over the 64 lanes of each pair
@@ -449,7 +530,7 @@ template <int K, int NP, bool FAST>
 __device__ __forceinline__ void store_codes(const FbArgs& a, const JobDev& j, const Lanes<NP>& L, int64_t ws,
                                             int64_t cnt, const float (&out)[NP][K])
 {
-    uint8_t* base = reinterpret_cast<uint8_t*>(j.mean);
+    global_char* base = (global_char*)j.mean;
 #pragma unroll
     for (int q = 0; q < NP; ++q) {
         if (!L.live[q]) continue;
@@ -459,7 +540,7 @@ __device__ __forceinline__ void store_codes(const FbArgs& a, const JobDev& j, co
             for (int k = 0; k < K; ++k)
                 if (FAST || k < cnt)
                     __builtin_nontemporal_store((uint8_t)log_code(out[q][k], a.q_inv_range, 255.0f),
-                                                base + (size_t)(ws + k - j.row0) * (size_t)j.P + col);
+                                                (uint8_t*)(base + (size_t)(ws + k - j.row0) * (size_t)j.P + col));
         } else {
 #pragma unroll
             for (int k = 0; k < K; k += 2) {
@@ -467,7 +548,7 @@ __device__ __forceinline__ void store_codes(const FbArgs& a, const JobDev& j, co
                     uint32_t v = log_code(out[q][k], a.q_inv_range, 15.0f);
                     if (FAST || k + 1 < cnt) v |= log_code(out[q][k + 1], a.q_inv_range, 15.0f) << 4;
                     __builtin_nontemporal_store((uint8_t)v,
-                                                base + (size_t)((ws + k - j.row0) >> 1) * (size_t)j.P + col);
+                                                (uint8_t*)(base + (size_t)((ws + k - j.row0) >> 1) * (size_t)j.P + col));
                 }
             }
         }
@@ -514,10 +595,16 @@ __device__ __forceinline__ void window_step(const FbArgs& a, const JobDev& j, co
 #pragma unroll
             for (int q = 0; q < NP; ++q) {
                 if (ST) {
-                    // next window: site ws - K + i, read back into the slot just freed
-                    const f32x2 f = get_state(ck_at(L, ws - K + i - cs, q));
-                    fm[q][slot] = f.x;
-                    fc[q][slot] = f.y;
+                    // next window: sites ws - K + i - 1 and ws - K + i, read back
+                    // into the two slots just freed
+                    if (i & 1) {
+                        const f32x4 f = get_rec(rec_at(L, (ws - K + i - 1 - cs) >> 1, q));
+                        const int prev = B ? i - 1 : K - i;
+                        fm[q][prev] = f.x;
+                        fc[q][prev] = f.y;
+                        fm[q][slot] = f.z;
+                        fc[q][slot] = f.w;
+                    }
                     continue;
                 }
                 if (FAST)
@@ -541,8 +628,20 @@ __device__ __forceinline__ void window_step(const FbArgs& a, const JobDev& j, co
         store_codes<K, NP, FAST>(a, j, L, ws, cnt, out);
         return;
     }
+    // row segments through LDS for full windows of full slots (float means
+    // only; the stores below otherwise)
+    const uintptr_t mb = reinterpret_cast<uintptr_t>(j.mean);
+    const int vec = (!CI && !POST && FAST) ? (((j.P & 3) == 0 && (mb & 15) == 0) ? 4 : ((j.P & 1) == 0 && (mb & 7) == 0) ? 2 : 1)
+                                          : 1;
 #pragma unroll
     for (int q = 0; q < NP; ++q) {
+        if (!CI && !POST && FAST && vec > 1 && (int64_t)(L.grp0 + q + 1) * 64 <= j.P) {
+            if (vec == 4)
+                store_means_rows<K, NP, 4>(j, L, q, ws, out[q], red);
+            else
+                store_means_rows<K, NP, 2>(j, L, q, ws, out[q], red);
+            continue;
+        }
         if (!L.live[q]) continue;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
@@ -625,6 +724,7 @@ __device__ __forceinline__ void forward_item(const FbArgs& a, int item, int lane
         m[q] = x0.x;
         c[q] = x0.y;
     }
+    HetCache<NP> hf, hb;
     for (int64_t s = ps; s < ce;) {
         const int64_t e = (s < cs) ? min(cs, s + K) : min(ce, s + K);
         const bool core = s >= cs;  // windows are all flank or all core
@@ -633,15 +733,26 @@ __device__ __forceinline__ void forward_item(const FbArgs& a, int item, int lane
             for (int q = 0; q < NP; ++q) *ck_at(L, (s - cs) / K, q) = make_float2(m[q], c[q]);
         }
         Window<K, NP> win;
-        load_window(a, L, s, win);
+        load_window(a, L, s, win, hf);
         if (s == ps) win.d[0] = b.sd_fwd;
+        // ST: core windows start an even number of sites after cs, so site
+        // pairs (k - 1, k), k odd, are records; a window's odd last site
+        // fills both halves of its record.
+        float pm[NP], pc[NP];
         if (e - s == K && no_long_gaps<K>(win.d)) {
 #pragma unroll
             for (int k = 0; k < K; ++k)
 #pragma unroll
                 for (int q = 0; q < NP; ++q) {
                     lookup(layer_of(a.t.fwd, win.d[k]), het_slab(win.bits[q], k), m[q], c[q]);
-                    if (ST && core) put_state(ck_at(L, s - cs + k, q), m[q], c[q]);
+                    if (ST && core) {
+                        if (k & 1) {
+                            put_rec(rec_at(L, (s - cs + k) >> 1, q), pm[q], pc[q], m[q], c[q]);
+                        } else {
+                            pm[q] = m[q];
+                            pc[q] = c[q];
+                        }
+                    }
                 }
         } else {
 #pragma unroll
@@ -650,7 +761,16 @@ __device__ __forceinline__ void forward_item(const FbArgs& a, int item, int lane
 #pragma unroll
                     for (int q = 0; q < NP; ++q) {
                         fwd_segment(a.t, win.d[k], het_slab(win.bits[q], k), m[q], c[q]);
-                        if (ST && core) put_state(ck_at(L, s - cs + k, q), m[q], c[q]);
+                        if (ST && core) {
+                            if (k & 1) {
+                                put_rec(rec_at(L, (s - cs + k) >> 1, q), pm[q], pc[q], m[q], c[q]);
+                            } else if (s + k + 1 == e) {
+                                put_rec(rec_at(L, (s - cs + k) >> 1, q), m[q], c[q], m[q], c[q]);
+                            } else {
+                                pm[q] = m[q];
+                                pc[q] = c[q];
+                            }
+                        }
                     }
         }
         s = e;
@@ -662,7 +782,7 @@ __device__ __forceinline__ void forward_item(const FbArgs& a, int item, int lane
     for (int64_t e = pe; e > ce;) {
         const int64_t b0 = max(ce, e - K);
         Window<K, NP> win;
-        load_window(a, L, b0, win);
+        load_window(a, L, b0, win, hb);
         if (e - b0 == K && no_long_gaps<K>(win.d)) {
 #pragma unroll
             for (int k = K - 1; k >= 0; --k)
@@ -707,23 +827,26 @@ __device__ __forceinline__ void backward_item(const FbArgs& a, int item, int lan
     float fm[NP][K], fc[NP][K];
     int64_t w = (int64_t)b.nwin - 1;
     int64_t cnt = ce - (cs + w * K);
+    HetCache<NP> hc, hn;  // current windows; next windows (recomputed forward states)
     {
         const int64_t ws = cs + w * K;
         if (ST) {
 #pragma unroll
-            for (int k = 0; k < K; ++k) {
+            for (int k = 0; k < K; k += 2) {
                 if (k < cnt) {
 #pragma unroll
                     for (int q = 0; q < NP; ++q) {
-                        const f32x2 f = get_state(ck_at(L, ws + k - cs, q));
+                        const f32x4 f = get_rec(rec_at(L, (ws + k - cs) >> 1, q));
                         fm[q][k] = f.x;
                         fc[q][k] = f.y;
+                        fm[q][k + 1] = f.z;
+                        fc[q][k + 1] = f.w;
                     }
                 }
             }
         } else {
             Window<K, NP> win;
-            load_window(a, L, ws, win);
+            load_window(a, L, ws, win, hn);
             if (ws == ps) win.d[0] = b.sd_fwd;
 #pragma unroll
             for (int q = 0; q < NP; ++q) {
@@ -749,10 +872,10 @@ __device__ __forceinline__ void backward_item(const FbArgs& a, int item, int lan
         const bool has_next = w > 0;
         const int64_t ws = cs + w * K;
         Window<K, NP> cur, nxt;
-        load_window(a, L, ws, cur);
+        load_window(a, L, ws, cur, hc);
         if (ws == ps) cur.d[0] = b.sd_bwd;
         if (has_next && !ST) {
-            load_window(a, L, ws - K, nxt);
+            load_window(a, L, ws - K, nxt, hn);
             if (ws - K == ps) nxt.d[0] = b.sd_fwd;
 #pragma unroll
             for (int q = 0; q < NP; ++q) {
@@ -794,6 +917,11 @@ __device__ __forceinline__ void backward_item(const FbArgs& a, int item, int lan
 // two), no floor otherwise.
 template <int NP, bool ST> constexpr int kBwdWaves = (NP == 1 && !ST) ? 3 : 1;
 
+// LDS floats of the backward kernel: the summary's staging rows, or the
+// [K][64] tile of float-mean row stores (reused slot by slot).
+template <int K, bool CI, bool POST, bool SUM>
+constexpr int kBwdLds = SUM ? 16 * 65 : (CI || POST) ? 1 : K * 64;
+
 // A decode is two launches over the same work items: fb_fwd_* (forward
 // walk + right-flank backward walk; few registers, so many waves per SIMD
 // to hide the latency of its single lookup chain) then fb_bwd_* (the
@@ -811,7 +939,7 @@ template <int K, int NP, bool CI, bool POST, bool SUM, bool ST>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kBwdWaves<NP, ST>))) void fb_bwd_items(FbArgs a)
 {
     const int item = __builtin_amdgcn_readfirstlane((int)blockIdx.x);
-    __shared__ float red[SUM ? 16 * 65 : 1];
+    __shared__ float red[kBwdLds<K, CI, POST, SUM>];
     if (item < a.n_items) backward_item<K, NP, CI, POST, SUM, ST>(a, item, threadIdx.x, red);
 }
 
@@ -832,7 +960,7 @@ __global__ __launch_bounds__(64) void fb_fwd_queue(FbArgs a)
 template <int K, int NP, bool CI, bool POST, bool SUM, bool ST>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kBwdWaves<NP, ST>))) void fb_bwd_queue(FbArgs a)
 {
-    __shared__ float red[SUM ? 16 * 65 : 1];
+    __shared__ float red[kBwdLds<K, CI, POST, SUM>];
     for (int item = next_item(a.queue2, threadIdx.x); item < a.n_items; item = next_item(a.queue2, threadIdx.x))
         backward_item<K, NP, CI, POST, SUM, ST>(a, item, threadIdx.x, red);
 }

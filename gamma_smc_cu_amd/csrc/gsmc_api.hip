@@ -558,7 +558,8 @@ Plan make_plan(const gsmc_context* ctx, const std::vector<JobSpec>& jobs, const 
             d.job = (int32_t)(jd.size() - 1);
             const int kw = stored ? 16 : window_k(tu);
             d.nwin = (int32_t)((b.ce - b.cs + kw - 1) / kw);
-            const int64_t rows = stored ? (int64_t)(b.ce - b.cs) : (int64_t)d.nwin;
+            // float2 rows: one per checkpoint window, or two per 16-byte record of two core sites
+            const int64_t rows = stored ? 2 * (int64_t)((b.ce - b.cs + 1) / 2) : (int64_t)d.nwin;
             require(bd.size() < (size_t)INT32_MAX, "too many block windows");
             // ck_off holds the item's state elements until phases place them
             for (int64_t t = 0; t < tiles; ++t) items.push_back({(int32_t)bd.size(), (int32_t)t, rows * tile_pairs});
@@ -743,9 +744,10 @@ FbArgs make_args(gsmc_batch* b, const Plan& pl, int flags)
     return a;
 }
 
-// The kernels of a planned launch, in order: the queue reset (persistent
-// schedule), the two decode kernels, the summary epilogues. No copies and
-// no memsets, so the sequence can also be captured into a graph.
+// The kernels of a planned launch, in order: per phase the queue reset
+// (persistent schedule) and the two decode kernels, then the summary
+// epilogues. No copies and no memsets, so the sequence can also be
+// captured into a graph.
 void enqueue_kernels(gsmc_context* ctx, const Plan& pl, const FbArgs& a, int flags, hipStream_t stream)
 {
     const Tuning& tu = ctx->tune;
@@ -1614,7 +1616,8 @@ int gsmc_checkpoint_bytes(gsmc_context* ctx, int64_t n_sites, int64_t core, int6
         const int64_t tiles = (n_pairs + tile_pairs - 1) / tile_pairs;
         uint64_t windows = 0;  // state rows: one per checkpoint window, or per core site with K = 1
         for (const Block& b : make_blocks(n_sites, core > 0 ? core : std::max<int64_t>(n_sites, 1), core > 0 ? flank : 0))
-            windows += (uint64_t)((b.ce - b.cs + ctx->tune.K - 1) / ctx->tune.K);
+            windows += ctx->tune.K == 1 ? (uint64_t)(2 * ((b.ce - b.cs + 1) / 2))
+                                        : (uint64_t)((b.ce - b.cs + ctx->tune.K - 1) / ctx->tune.K);
         *bytes = windows * (uint64_t)tiles * (uint64_t)tile_pairs * sizeof(float2);
     });
 }

@@ -213,8 +213,8 @@ def test_batch_matches_alone_and_oracle(core, cohort_small, flow_field_path, tu)
             assert _bits_equal(a[k], ref[k]), k
 
 
-@pytest.mark.parametrize("schedule", ["static", "persistent"])
-@pytest.mark.parametrize("budget", [0, 400_000])
+@pytest.mark.parametrize("schedule,budget", [("static", 0), ("persistent", 0), ("static", 400_000),
+                                             ("persistent", 400_000)])
 def test_stored_forward_states_bit_exact(core, cohort_small, flow_field_path, schedule, budget):
     """checkpoint_k=1: the forward kernel stores the state after every core
     site and the backward kernel reads it back instead of recomputing it.
@@ -452,17 +452,17 @@ def test_log_codes_match_oracle(core, cohort_small, flow_field_path, bits):
             ctx.run_fb_codes(pairs, bits=4, core_block_sites=201, flank_sites=10)
 
 
-@pytest.mark.parametrize("schedule", ["static", "persistent"])
-def test_graph_replays_match_eager(core, cohort_small, flow_field_path, schedule):
+@pytest.mark.parametrize("schedule,k", [("static", 16), ("persistent", 16), ("static", 1)])
+def test_graph_replays_match_eager(core, cohort_small, flow_field_path, schedule, k):
     """A captured batched launch, replayed several times with other eager
     launches and synchronisations in between, reproduces the eager launch
-    bit for bit on every replay."""
+    bit for bit on every replay (also with stored forward states)."""
     from gamma_smc_cu_amd.batch import Graph, Staged, Workspace
     from gamma_smc_cu_amd.device import Stream
 
     G, pos = cohort_small
     ctx = core.FlowContext(G, pos, 1e4, 1.25e-8, 1e-8, flow_field_path, 0)
-    ctx.set_tuning(checkpoint_k=16, pairs_per_lane=1, schedule=schedule)
+    ctx.set_tuning(checkpoint_k=k, pairs_per_lane=1, schedule=schedule)
     jobs = [j for j in _ragged_jobs(G.shape[0], G.shape[1], np.random.default_rng(5)) if j.n_sites > 0]
     st = Staged(jobs)
     arr, n = st.structs()
