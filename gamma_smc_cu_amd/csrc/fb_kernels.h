@@ -126,6 +126,7 @@ struct FbArgs {
     float2* ck;                // checkpoints: per item, [nwin][64 NP]
     float2* yb;                // backward state at each item's core end: per item, [64 NP]
     int* queue2;               // persistent mode: item counter of the second kernel
+    int64_t slot_elems;        // fused stored-state kernel: float2 state elements of each wave's slot
     DevTables t;
     float unscale;             // 2 Ne
     const float2* prior;       // prior[q] = q full N-bp hom stretches applied to the prior (0, 0)
@@ -663,7 +664,7 @@ template <int NP> struct ItemView {
 };
 
 template <int NP>
-__device__ __forceinline__ ItemView<NP> open_item(const FbArgs& a, int item_in, int lane)
+__device__ __forceinline__ ItemView<NP> open_item(const FbArgs& a, int item_in, int lane, int slot)
 {
     ItemView<NP> v;
     // The item is the same for the whole wave; saying so keeps every piece
@@ -689,8 +690,10 @@ __device__ __forceinline__ ItemView<NP> open_item(const FbArgs& a, int item_in, 
         v.L.ri[q] = a.packed + (size_t)hi * (size_t)a.W;
         v.L.rj[q] = a.packed + (size_t)hj * (size_t)a.W;
     }
-    // Every lane, idle ones included, owns its state columns.
-    v.L.ck = a.ck + it.ck_off;
+    // Every lane, idle ones included, owns its state columns: the item's
+    // rows of the launch's state buffer, or (slot >= 0) the rows of the
+    // wave's own slot, reused by each item the wave decodes.
+    v.L.ck = slot >= 0 ? a.ck + (size_t)slot * (size_t)a.slot_elems : a.ck + it.ck_off;
     v.L.ck_col = (uint32_t)lane * 8u;
     v.L.grp0 = it.tile * NP;
     return v;
@@ -706,9 +709,9 @@ template <int NP> __device__ __forceinline__ float2* yb_at(const FbArgs& a, int 
 // the state after every core site), and the backward walk over the right
 // flank [ce, pe), whose final state is the backward state entering the core.
 template <int K, int NP, bool ST>
-__device__ __forceinline__ void forward_item(const FbArgs& a, int item, int lane)
+__device__ __forceinline__ void forward_item(const FbArgs& a, int item, int lane, int slot = -1)
 {
-    const ItemView<NP> v = open_item<NP>(a, item, lane);
+    const ItemView<NP> v = open_item<NP>(a, item, lane, slot);
     if (!v.valid) return;
     const BlockDev& b = v.b;
     const Lanes<NP>& L = v.L;
@@ -807,9 +810,9 @@ __device__ __forceinline__ void forward_item(const FbArgs& a, int item, int lane
 // forward states are read back instead of recomputed, so the next
 // window's descriptors and het bits are not needed.
 template <int K, int NP, bool CI, bool POST, bool SUM, bool ST>
-__device__ __forceinline__ void backward_item(const FbArgs& a, int item, int lane, float* red)
+__device__ __forceinline__ void backward_item(const FbArgs& a, int item, int lane, float* red, int slot = -1)
 {
-    const ItemView<NP> v = open_item<NP>(a, item, lane);
+    const ItemView<NP> v = open_item<NP>(a, item, lane, slot);
     if (!v.valid) return;
     const BlockDev& b = v.b;
     const JobDev& j = v.j;
@@ -963,6 +966,26 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kBwdWaves<NP
     __shared__ float red[kBwdLds<K, CI, POST, SUM>];
     for (int item = next_item(a.queue2, threadIdx.x); item < a.n_items; item = next_item(a.queue2, threadIdx.x))
         backward_item<K, NP, CI, POST, SUM, ST>(a, item, threadIdx.x, red);
+}
+
+// Stored forward states, one launch: persistent waves take items from the
+// device counter and decode each whole -- the forward half into the wave's
+// slot of the state buffer, then the backward half reading it back -- so
+// the state buffer holds one item per resident wave, not one per item, and
+// one pool of waves drains all items (no second kernel waiting on the
+// first one's last wave). The fence makes the slot's states and the
+// backward entry state written by the forward half visible to the
+// backward half's loads.
+template <int KF, int KB, int NP, bool SUM, int WAVES>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES))) void fb_fused_queue(FbArgs a)
+{
+    __shared__ float red[kBwdLds<KB, false, false, SUM>];
+    const int slot = (int)blockIdx.x;
+    for (int item = next_item(a.queue, threadIdx.x); item < a.n_items; item = next_item(a.queue, threadIdx.x)) {
+        forward_item<KF, NP, true>(a, item, threadIdx.x, slot);
+        __threadfence();
+        backward_item<KB, NP, false, false, SUM, true>(a, item, threadIdx.x, red, slot);
+    }
 }
 
 // Summary epilogue: per site r of a job, the groups' partials in group

@@ -480,15 +480,28 @@ struct Plan {
     size_t part_elems = 0;        // summary partials (per plane)
     std::vector<SumJob> sums;
     std::vector<Phase> phases;
+    int64_t fused_grid = 0;   // stored states: resident waves of the fused kernel
+    int64_t slot_elems = 0;   // ... and the float2 state elements of each wave's slot
 };
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // Forward states stored at every core site (instead of checkpoints every K
 // sites and a recompute): tuning K = 1, for the mean / code / summary
-// kernels. Those kernels walk windows of 16 sites.
+// kernels. Those run as one fused persistent kernel whose forward half
+// walks windows of 16 sites and backward half windows of kStoredBwdK (short
+// windows keep the kernel at six waves per SIMD).
 bool stored_states(const Tuning& tu, int flags) { return tu.K == 1 && !(flags & (GSMC_CI | GSMC_POSTERIOR)); }
 int window_k(const Tuning& tu) { return tu.K == 8 ? 8 : 16; }
+constexpr int kStoredBwdK = 4;
+constexpr int kStoredWaves = 6;
+
+const void* fused_kernel(int flags)
+{
+    return (flags & GSMC_SUMMARY) ? reinterpret_cast<const void*>(&fb_fused_queue<16, kStoredBwdK, 1, true, kStoredWaves>)
+                                  : reinterpret_cast<const void*>(&fb_fused_queue<16, kStoredBwdK, 1, false, kStoredWaves>);
+}
+int64_t resident(const gsmc_context* ctx, const void* fn);
 
 // Bytes the state buffer of workspace b may take for a plan needing `want`:
 // all of it when that fits in the current allocation or in the allocation
@@ -505,7 +518,7 @@ size_t state_budget(const gsmc_context* ctx, size_t have, size_t want)
 }
 
 Plan make_plan(const gsmc_context* ctx, const std::vector<JobSpec>& jobs, const Tuning& tu, int np,
-               bool summary, bool stored, size_t have_ck_bytes)
+               bool summary, bool stored, size_t have_ck_bytes, int flags)
 {
     std::vector<JobDev> jd;
     std::vector<BlockDev> bd;
@@ -556,7 +569,7 @@ Plan make_plan(const gsmc_context* ctx, const std::vector<JobSpec>& jobs, const 
                 d.sd_bwd = 0;
             }
             d.job = (int32_t)(jd.size() - 1);
-            const int kw = stored ? 16 : window_k(tu);
+            const int kw = stored ? kStoredBwdK : window_k(tu);
             d.nwin = (int32_t)((b.ce - b.cs + kw - 1) / kw);
             // float2 rows: one per checkpoint window, or two per 16-byte record of two core sites
             const int64_t rows = stored ? 2 * (int64_t)((b.ce - b.cs + 1) / 2) : (int64_t)d.nwin;
@@ -580,6 +593,36 @@ Plan make_plan(const gsmc_context* ctx, const std::vector<JobSpec>& jobs, const 
         };
         return cost(x) > cost(y);
     });
+    if (stored) {
+        // one fused launch: a state slot (the largest item's rows) per
+        // resident wave, as many waves as fit in the budget
+        size_t slot = 0;
+        for (const ItemDev& it : items) slot = std::max(slot, (size_t)it.ck_off);
+        slot = std::max<size_t>(slot, 1);
+        const int64_t want = std::max<int64_t>(1, std::min<int64_t>(resident(ctx, fused_kernel(flags)),
+                                                                    (int64_t)items.size()));
+        const size_t budget = state_budget(ctx, have_ck_bytes, (size_t)want * slot * sizeof(float2)) / sizeof(float2);
+        if (slot > budget)
+            throw Error(GSMC_ENOMEM, "state budget below the forward states of one work item (" +
+                                         std::to_string(slot * sizeof(float2)) + " bytes)");
+        pl.fused_grid = std::min<int64_t>(want, (int64_t)(budget / slot));
+        pl.slot_elems = (int64_t)slot;
+        for (ItemDev& it : items) it.ck_off = 0;
+        pl.phases.push_back({0, (int64_t)items.size()});
+        pl.n_items = (int64_t)items.size();
+        pl.ck_elems = (size_t)pl.fused_grid * slot;
+        pl.part_elems = part;
+        pl.sums = std::move(sums);
+        const size_t sj = jd.size() * sizeof(JobDev), sb = bd.size() * sizeof(BlockDev);
+        pl.off_blocks = align256(sj);
+        pl.off_items = pl.off_blocks + align256(sb);
+        pl.bytes.assign(pl.off_items + items.size() * sizeof(ItemDev), 0);
+        if (sj) std::memcpy(pl.bytes.data(), jd.data(), sj);
+        if (sb) std::memcpy(pl.bytes.data() + pl.off_blocks, bd.data(), sb);
+        if (!items.empty())
+            std::memcpy(pl.bytes.data() + pl.off_items, items.data(), items.size() * sizeof(ItemDev));
+        return pl;
+    }
     // Phases: when the states of all items exceed the budget, deal the
     // items round-robin (longest first) into the fewest phases that fit, so
     // every phase gets the same mix of long and short items.
@@ -680,9 +723,6 @@ template <int K> KernelPair kernels_k(const Tuning& tu, int flags)
 
 KernelPair kernels_for(const Tuning& tu, int flags)
 {
-    if (stored_states(tu, flags))
-        return (flags & GSMC_SUMMARY) ? kernels_of<16, 1, false, false, true, true>()
-                                      : kernels_of<16, 1, false, false, false, true>();
     return window_k(tu) == 8 ? kernels_k<8>(tu, flags) : kernels_k<16>(tu, flags);
 }
 
@@ -729,6 +769,7 @@ FbArgs make_args(gsmc_batch* b, const Plan& pl, int flags)
     a.n_items = pl.n_items;
     a.queue = b->queue.as<int>();
     a.queue2 = b->queue.as<int>() + 1;
+    a.slot_elems = pl.slot_elems;
     a.ck = b->ck.as<float2>();
     a.yb = b->yb.as<float2>();
     a.t = ctx->tabs->view;
@@ -751,8 +792,13 @@ FbArgs make_args(gsmc_batch* b, const Plan& pl, int flags)
 void enqueue_kernels(gsmc_context* ctx, const Plan& pl, const FbArgs& a, int flags, hipStream_t stream)
 {
     const Tuning& tu = ctx->tune;
+    if (pl.fused_grid > 0) {
+        hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(1), 0, stream, a.queue);
+        launch(fused_kernel(flags), (unsigned)pl.fused_grid, a, stream);
+    }
     const KernelPair kp = kernels_for(tu, flags);
     for (const Phase& ph : pl.phases) {
+        if (pl.fused_grid > 0) break;
         if (ph.count == 0) continue;
         FbArgs pa = a;
         pa.items = a.items + ph.first;
@@ -785,7 +831,7 @@ void launch_plan(gsmc_batch* b, const std::vector<JobSpec>& jobs, int flags, hip
     gsmc_context* ctx = b->ctx;
     const Tuning tu = ctx->tune;
     const bool summary = (flags & GSMC_SUMMARY) != 0;
-    Plan pl = make_plan(ctx, jobs, tu, launch_np(tu, flags), summary, stored_states(tu, flags), b->ck.bytes);
+    Plan pl = make_plan(ctx, jobs, tu, launch_np(tu, flags), summary, stored_states(tu, flags), b->ck.bytes, flags);
     if (pl.n_items == 0) return;
     const bool same = b->pending && stream == b->last_stream && pl.bytes == b->last;
     if (!same) {
@@ -1489,7 +1535,7 @@ int gsmc_batch_capture(gsmc_context* ctx, const gsmc_job* jobs, int64_t n_jobs, 
         g->buf = new_batch(ctx);
         const bool summary = (flags & GSMC_SUMMARY) != 0;
         Plan pl = make_plan(ctx, specs, ctx->tune, launch_np(ctx->tune, flags), summary,
-                            stored_states(ctx->tune, flags), 0);
+                            stored_states(ctx->tune, flags), 0, flags);
         require(pl.n_items > 0, "nothing to capture: every job is empty");
         ensure_buffers(g->buf.get(), pl, summary);
         HIP_OK(hipMemcpy(g->buf->meta.p, pl.bytes.data(), pl.bytes.size(), hipMemcpyHostToDevice));

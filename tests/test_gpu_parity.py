@@ -214,13 +214,14 @@ def test_batch_matches_alone_and_oracle(core, cohort_small, flow_field_path, tu)
 
 
 @pytest.mark.parametrize("schedule,budget", [("static", 0), ("persistent", 0), ("static", 400_000),
-                                             ("persistent", 400_000)])
+                                             ("persistent", 800_000)])
 def test_stored_forward_states_bit_exact(core, cohort_small, flow_field_path, schedule, budget):
-    """checkpoint_k=1: the forward kernel stores the state after every core
-    site and the backward kernel reads it back instead of recomputing it.
-    Means, log codes and ragged batches equal the oracle's bit for bit,
-    also when a state budget of about one work item (700 sites x 64 pairs
-    x 8 B = 358,400 B) splits each launch into many phases."""
+    """checkpoint_k=1: each wave's forward walk stores the state after
+    every core site in the wave's slot and its backward walk reads it back
+    instead of recomputing it. Means, log codes and ragged batches equal the
+    oracle's bit for bit, also when a state budget of about one slot of the
+    largest work item (700 sites x 64 pairs x 8 B = 358,400 B) leaves a
+    single wave, or two, to decode every item in turn."""
     from gamma_smc_cu_amd import batch
 
     G, pos = cohort_small
@@ -263,7 +264,7 @@ def test_state_budget_below_one_item_raises(core, cohort_small, flow_field_path)
     ctx = core.FlowContext(G, pos, 1e4, 1.25e-8, 1e-8, flow_field_path, 0)
     ctx.set_tuning(checkpoint_k=1)
     ctx.set_state_budget(1000)
-    with pytest.raises(MemoryError, match="one work item"):
+    with pytest.raises(MemoryError, match="work item"):
         ctx.run_fb([(1, 0)], mean_only=True)
     ctx.set_state_budget(0)
     assert ctx.run_fb([(1, 0)], mean_only=True)["mean"].shape == (G.shape[1], 1)
