@@ -39,7 +39,8 @@ HBM_PEAK_GBS = 8000.0
 # commands is profiles/fb_decode_chromosome_counters.md. Reported as
 # roofline.traffic only when a run matches that workload and shape.
 MEASURED_TRAFFIC = {
-    ("chromosome", 16, 1, "static", False): 151.49e9,
+    ("chromosome", 8, 1, "static", False): 186.96e9,
+    ("chromosome", 16, 1, "static", False): 155.15e9,
 }
 
 WORKLOADS = {
