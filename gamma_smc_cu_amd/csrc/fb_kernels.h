@@ -1039,13 +1039,20 @@ __global__ void prior_table_kernel(DevTables t, float2* out, int64_t n)
 
 // ---------------------------------------------------------------------------
 // Forward-only filter (run_fwd): the reference's half-precision cached path,
-// gamma_smc_flow.cu:1139-1232. Tables: the composed transitions cache[k]
-// (N layers of 2550 (m, c) nodes) stored as FP16 pairs. Per site: advance by
-// the site's gap (descriptor kNoAdvance at site 0 and for gaps that round
+// gamma_smc_flow.cu:1139-1257. Tables: the composed transitions cache[k]
+// (N layers of 2550 (m, c) nodes) rounded to FP16 pairs. Per site: advance
+// by the site's gap (descriptor kNoAdvance at site 0 and for gaps that round
 // to 0; full N-bp layers first), apply the het emission explicitly, output
-// 10^m * 2Ne (and the Wilson-Hilferty bounds). One lane per pair; outputs of
-// each 16-site window are buffered and stored after it.
+// 10^m * 2Ne (and the Wilson-Hilferty bounds). One lane per pair, so a
+// launch has only P / 64 waves and its time is the per-site latency of the
+// dependent chain times the sites: one 16-byte gather per lookup and the
+// fast 10^x keep that chain short.
 constexpr uint32_t kNoAdvance = 0xFFFFFFFFu;
+
+// FP16 slab layout: entry (r, q) (r < 50, q < 50) holds the four corners
+// (r, q), (r + 1, q), (r, q + 1), (r + 1, q + 1) as FP16 (m, c) pairs, 16
+// bytes, so a lookup is one aligned gather. 40,000 bytes per layer.
+constexpr uint32_t kQuadSlabEntries = 50 * 50;
 
 struct FwdArgs {
     const uint64_t* packed;
@@ -1053,7 +1060,7 @@ struct FwdArgs {
     int64_t W;
     int64_t S;
     const uint32_t* desc;   // [S + 64] forward-only gap descriptors
-    const __half2* h2;      // [N][2550] (m, c)
+    const uint4* h2q;       // [N][50 * 50] quad entries
     int32_t N;
     const int32_t* pair_i;
     const int32_t* pair_j;
@@ -1064,7 +1071,14 @@ struct FwdArgs {
     float* upper;
 };
 
-__device__ __forceinline__ void lookup_h2(const __half2* __restrict__ layer, float& m, float& c)
+__device__ __forceinline__ float2 half2_bits(uint32_t v)
+{
+    return __half22float2(*reinterpret_cast<const __half2*>(&v));
+}
+
+// Bilinear lookup on one FP16 slab, the arithmetic of the FP32 lookup
+// (gamma_smc_flow.cu:1139-1172).
+__device__ __forceinline__ void lookup_h2(const uint4* __restrict__ layer, float& m, float& c)
 {
     float fm = (m - -5.0f) * kMeanStepInv;
     float fc = (c - -2.0f) * kCvStepInv;
@@ -1072,9 +1086,8 @@ __device__ __forceinline__ void lookup_h2(const __half2* __restrict__ layer, flo
     fc = __builtin_amdgcn_fmed3f(fc, 0.0f, 49.0f);
     const float lm = fminf(floorf(fm), 49.0f), lc = fminf(floorf(fc), 48.0f);
     const float wm = fm - lm, wc = fc - lc;
-    const int base = (int)lm * 50 + (int)lc;
-    const float2 v00 = __half22float2(layer[base]), v01 = __half22float2(layer[base + 1]);
-    const float2 v10 = __half22float2(layer[base + 50]), v11 = __half22float2(layer[base + 51]);
+    const uint4 e = layer[(uint32_t)fmaf(lm, 50.0f, lc)];
+    const float2 v00 = half2_bits(e.x), v10 = half2_bits(e.y), v01 = half2_bits(e.z), v11 = half2_bits(e.w);
     const float omwm = 1.0f - wm, omwc = 1.0f - wc;
     m = fmaf(wm, fmaf(wc, v11.x, omwc * v10.x), omwm * fmaf(wc, v01.x, omwc * v00.x));
     c = fmaf(wm, fmaf(wc, v11.y, omwc * v10.y), omwm * fmaf(wc, v01.y, omwc * v00.y));
@@ -1088,14 +1101,24 @@ __global__ __launch_bounds__(64) void fwd_only_kernel(FwdArgs a)
     const int64_t pp = live ? p : a.P - 1;
     const int64_t hi = min(max((int64_t)a.pair_i[pp], (int64_t)0), a.n_haps - 1);
     const int64_t hj = min(max((int64_t)a.pair_j[pp], (int64_t)0), a.n_haps - 1);
-    const uint64_t* ri = a.packed + (size_t)hi * (size_t)a.W;
-    const uint64_t* rj = a.packed + (size_t)hj * (size_t)a.W;
-    const __half2* last = a.h2 + (size_t)(a.N - 1) * kGridNodes;
+    Lanes<1> L;
+    L.ri[0] = a.packed + (size_t)hi * (size_t)a.W;
+    L.rj[0] = a.packed + (size_t)hj * (size_t)a.W;
+    HetCache<1> h;
+    const uint4* last = a.h2q + (size_t)(a.N - 1) * kQuadSlabEntries;
     float m = 0.0f, c = 0.0f;
     for (int64_t s0 = 0; s0 < a.S; s0 += 16) {
         uint32_t d[16];
         load_descs<16>(a.desc, s0, d);
-        const uint64_t bits = het_mask(ri, rj, s0, a.W);
+        // het words as the decode kernels cache them (one word pair per 64 sites)
+        const int64_t w = s0 >> 6;
+        if (w != h.w) {
+            h.lo[0] = het_word(L, 0, w, a.W);
+            h.hi[0] = het_word(L, 0, w + 1, a.W);
+            h.w = w;
+        }
+        const int sh = (int)(s0 & 63);
+        const uint64_t bits = sh == 0 ? h.lo[0] : (h.lo[0] >> sh) | (h.hi[0] << (64 - sh));
         const int64_t cnt = min((int64_t)16, a.S - s0);
         float out[16], olo[16], ohi[16];
 #pragma unroll
@@ -1104,11 +1127,12 @@ __global__ __launch_bounds__(64) void fwd_only_kernel(FwdArgs a)
                 const uint32_t dk = d[k];
                 if (dk != kNoAdvance) {
                     for (uint32_t q = dk >> kDescLayerBits; q != 0; --q) lookup_h2(last, m, c);
-                    lookup_h2(a.h2 + (size_t)(dk & kDescLayerMask) * kGridNodes, m, c);
+                    lookup_h2(a.h2q + (size_t)(dk & kDescLayerMask) * kQuadSlabEntries, m, c);
                 }
                 if (het_bit(bits, k)) {
-                    const float alpha = exp10f(-2.0f * c) + 1.0f;
-                    const float a_log = log10f(alpha);
+                    // the reference's fast 10^x and log10 (gamma_smc_flow.cu:1234-1235)
+                    const float alpha = __exp10f(-2.0f * c) + 1.0f;
+                    const float a_log = __log10f(alpha);
                     const float b_log = fmaf(-2.0f, c, -m);
                     m = a_log - b_log;
                     c = -0.5f * a_log;
@@ -1116,7 +1140,7 @@ __global__ __launch_bounds__(64) void fwd_only_kernel(FwdArgs a)
                 const float mg = exp2f(m * 3.321928094887362f) * a.two_ne;
                 out[k] = mg;
                 if (CI) {
-                    const float alpha = exp10f(-2.0f * c);
+                    const float alpha = __exp10f(-2.0f * c);
                     const float inv9a = 1.0f / (9.0f * alpha);
                     const float sq = sqrtf(inv9a);
                     const float base = 1.0f - inv9a;
@@ -1143,11 +1167,24 @@ __global__ __launch_bounds__(64) void fwd_only_kernel(FwdArgs a)
     }
 }
 
-// FP32 (m, c) pairs to FP16 pairs, round to nearest even.
-__global__ void to_half2_kernel(const float2* __restrict__ src, __half2* __restrict__ dst, int64_t n)
+// FP32 (m, c) nodes [N][51][50] to FP16 quad entries [N][50][50] (round to
+// nearest even, as the reference rounds its table with __floats2half2_rn):
+// entry (r, q) = nodes (r, q), (r + 1, q), (r, q + 1), (r + 1, q + 1); the
+// q + 1 of the unused last column repeats q.
+__global__ void to_half2_quads_kernel(const float2* __restrict__ src, uint4* __restrict__ dst, int64_t n_layers)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = __floats2half2_rn(src[i].x, src[i].y);
+    if (i >= n_layers * (int64_t)kQuadSlabEntries) return;
+    const int64_t layer = i / kQuadSlabEntries;
+    const int e = (int)(i - layer * kQuadSlabEntries);
+    const int r = e / 50, q = e % 50, q1 = q < 49 ? q + 1 : q;
+    const float2* nodes = src + layer * kGridNodes;
+    auto bits = [](float2 v) {
+        const __half2 h = __floats2half2_rn(v.x, v.y);
+        return *reinterpret_cast<const uint32_t*>(&h);
+    };
+    dst[i] = make_uint4(bits(nodes[r * 50 + q]), bits(nodes[(r + 1) * 50 + q]), bits(nodes[r * 50 + q1]),
+                        bits(nodes[(r + 1) * 50 + q1]));
 }
 
 __global__ void reset_counters_kernel(int* counters)

@@ -127,12 +127,12 @@ struct DeviceTables {
     char* base = nullptr;  // fwd layers, bwd layers, stretch slab
     size_t bytes = 0;
     DevTables view{};
-    __half2* h2 = nullptr;  // forward-only path: cache[k] as FP16 (m, c), [N][2550]
+    uint4* h2q = nullptr;   // forward-only path: cache[k] as FP16 quad entries, [N][50 * 50]
     ~DeviceTables()
     {
-        if (base || h2) (void)hipSetDevice(device);
+        if (base || h2q) (void)hipSetDevice(device);
         if (base) (void)hipFree(base);
-        if (h2) (void)hipFree(h2);
+        if (h2q) (void)hipFree(h2q);
     }
 };
 
@@ -213,14 +213,16 @@ std::shared_ptr<DeviceTables> get_tables(int device, const char* path, int N, do
     dt->view.stretch = dt->base + 2 * layers;
     {
         // FP16 copy of the composed transitions, rounded on the device as
-        // the reference rounds them on the host (__floats2half2_rn).
+        // the reference rounds them on the host (__floats2half2_rn), laid
+        // out as quad entries (one 16-byte gather per lookup).
         const int64_t nodes = (int64_t)N * kGridNodes;
+        const int64_t quads = (int64_t)N * kQuadSlabEntries;
         float2* tmp = nullptr;
         HIP_OK(hipMalloc(&tmp, (size_t)nodes * sizeof(float2)));
-        HIP_OK(hipMalloc(&dt->h2, (size_t)nodes * sizeof(__half2)));
+        HIP_OK(hipMalloc(&dt->h2q, (size_t)quads * sizeof(uint4)));
         HIP_OK(hipMemcpy(tmp, host.cache.data(), (size_t)nodes * sizeof(float2), hipMemcpyHostToDevice));
-        hipLaunchKernelGGL(to_half2_kernel, dim3((unsigned)((nodes + 255) / 256)), dim3(256), 0, nullptr, tmp, dt->h2,
-                           nodes);
+        hipLaunchKernelGGL(to_half2_quads_kernel, dim3((unsigned)((quads + 255) / 256)), dim3(256), 0, nullptr, tmp,
+                           dt->h2q, (int64_t)N);
         HIP_OK(hipGetLastError());
         HIP_OK(hipDeviceSynchronize());
         HIP_OK(hipFree(tmp));
@@ -1392,7 +1394,7 @@ void launch_fwd_only(gsmc_context* ctx, const int32_t* d_pi, const int32_t* d_pj
     a.W = ctx->W;
     a.S = ctx->S;
     a.desc = ctx->desc_h2.as<uint32_t>();
-    a.h2 = ctx->tabs->h2;
+    a.h2q = ctx->tabs->h2q;
     a.N = ctx->N;
     a.pair_i = d_pi;
     a.pair_j = d_pj;
