@@ -452,18 +452,20 @@ __device__ __forceinline__ void store_means_rows(const JobDev& j, const Lanes<NP
 // Per-site reduction of one window's means over the 64 lanes of each pair
 // group, for summary launches. The means go through LDS (one row of 65
 // floats per site, so both the writes and the reads below are free of bank
-// conflicts); lane (k = lane / 4, p = lane % 4) then sums lanes 16p .. 16p+15
-// of site k in order, in FP64 from 0.0, and a butterfly over p (partner
-// p ^ 1, then p ^ 2) joins the four parts. Lanes without a pair hold 0 and
-// are left out of min / max. The tree is the same for every K, so results
-// do not depend on the checkpoint interval. One lane per site stores the
-// group's partials.
+// conflicts); lane (k = lane % 16, p = lane / 16) then sums lanes 16p ..
+// 16p+15 of site k in order, in FP64 from 0.0, and a butterfly over p
+// (partner p ^ 1, then p ^ 2) joins the four parts. Lanes without a pair
+// hold 0 and are left out of min / max. The tree is the same for every K,
+// so results do not depend on the checkpoint interval. One lane per site
+// stores the group's partials.
 template <int K, int NP>
 __device__ __forceinline__ void summarize(const FbArgs& a, const JobDev& j, const Lanes<NP>& L, int64_t ws,
                                           int64_t cnt, const float (&out)[NP][K], float* red)
 {
     const int lane = (int)threadIdx.x;
-    const int k = lane >> 2, part = lane & 3;
+    // lanes 16 p .. 16 p + 15 read part p of rows 0..15: within each 32-lane
+    // half of a read, rows hit banks k and parts banks 16 p, all distinct
+    const int k = lane & 15, part = lane >> 4;
 #pragma unroll
     for (int q = 0; q < NP; ++q) {
         // pairs of this group that exist: a prefix of the 64 lanes
@@ -498,7 +500,7 @@ __device__ __forceinline__ void summarize(const FbArgs& a, const JobDev& j, cons
             }
         }
 #pragma unroll
-        for (int mask = 1; mask <= 2; mask <<= 1) {
+        for (int mask = 16; mask <= 32; mask <<= 1) {  // parts p ^ 1, then p ^ 2
             sum = sum + __shfl_xor(sum, mask);
             lo = fminf(lo, __shfl_xor(lo, mask));
             hi = fmaxf(hi, __shfl_xor(hi, mask));
