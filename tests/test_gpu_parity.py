@@ -259,6 +259,28 @@ def test_stored_forward_states_bit_exact(core, cohort_small, flow_field_path, sc
         assert _bits_equal(a["mean"], ref["mean"])
 
 
+@pytest.mark.parametrize("schedule", ["static", "persistent"])
+def test_checkpoint_phases_bit_exact(core, cohort_small, flow_field_path, schedule):
+    """A state budget of about one work item's checkpoints (700 sites / 8 x
+    64 pairs x 8 B = 44,800 B) splits checkpoint launches into consecutive
+    phases of items that reuse the buffer; every output stays the oracle's,
+    bit for bit, with CI and posterior too."""
+    G, pos = cohort_small
+    pairs = O.all_pairs(G.shape[0]) + [(3, 3)]
+    T = _tables(flow_field_path, 1.25e-8, 1e-8)
+    ctx = core.FlowContext(G, pos, 1e4, 1.25e-8, 1e-8, flow_field_path, 0)
+    ctx.set_tuning(checkpoint_k=8, pairs_per_lane=1, schedule=schedule)
+    ctx.set_state_budget(50_000)
+    _case(f"checkpoint phases {schedule}: full sequence")
+    got = ctx.run_fb(pairs, mean_only=False, return_posterior=True)
+    ref = O.fb_window_f32(G, pos, pairs, T, 1e4, ci=True, posterior=True)
+    for k, v in ref.items():
+        assert _bits_equal(got[k], v), k
+    _case(f"checkpoint phases {schedule}: blockwise")
+    got = ctx.run_fb_blockwise(pairs, core_block_sites=200, flank_sites=64, pair_batch_size=-1, max_streams=1)
+    assert _bits_equal(got["mean"], O.fb_blockwise_f32(G, pos, pairs, T, 1e4, 200, 64)["mean"])
+
+
 def test_state_budget_below_one_item_raises(core, cohort_small, flow_field_path):
     G, pos = cohort_small
     ctx = core.FlowContext(G, pos, 1e4, 1.25e-8, 1e-8, flow_field_path, 0)
