@@ -44,7 +44,7 @@ EXPORTS = (
     "gsmc_batch_query", "gsmc_checkpoint_bytes", "gsmc_device_copy", "gsmc_memcpy_htod_async",
     "gsmc_memcpy_dtoh_async", "gsmc_host_alloc", "gsmc_host_free", "gsmc_event_query",
     "gsmc_batch_reserve", "gsmc_run_fb_summary", "gsmc_run_fwd", "gsmc_run_fwd_device", "gsmc_batch_capture",
-    "gsmc_graph_launch", "gsmc_graph_destroy", "gsmc_context_set_state_budget",
+    "gsmc_graph_launch", "gsmc_graph_destroy", "gsmc_context_set_state_budget", "gsmc_context_release_buffers",
 )
 
 
@@ -84,6 +84,7 @@ def _declare(L):
                               ctypes.POINTER(ctypes.c_float)],
         "gsmc_context_set_tuning": [_vp, _i32, _i32, _i32, _i32, _i64],
         "gsmc_context_set_state_budget": [_vp, ctypes.c_uint64],
+        "gsmc_context_release_buffers": [_vp],
         "gsmc_run_fb": [_vp, _vp, _vp, _i64, _i32, _vp, _vp, _vp, _vp, _vp],
         "gsmc_block_windows": [_i64, _i64, _i64, _vp, _i64, ctypes.POINTER(_i64)],
         "gsmc_run_fb_blockwise": [_vp, _vp, _vp, _i64, _i64, _i64, _i64, _i32, _i32, _vp, _vp, _vp, _vp, _vp],
@@ -340,6 +341,11 @@ class FlowContext:
         sched = {"static": 0, "persistent": 1}[schedule]
         check(lib().gsmc_context_set_tuning(self._h, int(checkpoint_k), int(pairs_per_lane), sched,
                                             int(bool(xcd_order)), int(min_waves_per_launch)))
+
+    def release_buffers(self):
+        """Free the device workspaces this context keeps between calls (they
+        keep their largest launch's buffers); the next call allocates again."""
+        check(lib().gsmc_context_release_buffers(self._h))
 
     def set_state_budget(self, max_bytes=0):
         """Cap the device memory a launch uses for checkpoints / stored

@@ -47,6 +47,9 @@ def tune(ctx, run_once, candidates=None, repeats=1, log=None):
             log(f"tune {cand} -> {ms:.2f} ms")
     best = min(results, key=lambda r: r[1])[0]
     ctx.set_tuning(**best)
+    # the sweep leaves the workspaces sized for its largest candidate (the
+    # stored-state shape holds a state slot per resident wave)
+    ctx.release_buffers()
     return best, results
 
 

@@ -1211,6 +1211,31 @@ int gsmc_context_set_tuning(gsmc_context* ctx, int k, int pairs_per_lane, int sc
     });
 }
 
+int gsmc_context_release_buffers(gsmc_context* ctx)
+{
+    return guarded([&] {
+        require(ctx != nullptr, "context is NULL");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        HIP_OK(hipSetDevice(ctx->device));
+        if (ctx->compute) HIP_OK(hipStreamSynchronize(ctx->compute));
+        if (ctx->copy) HIP_OK(hipStreamSynchronize(ctx->copy));
+        gsmc_batch* own[3] = {ctx->dev_batch.get(), ctx->host_batch[0].get(), ctx->host_batch[1].get()};
+        for (gsmc_batch* b : own) {
+            if (!b) continue;
+            if (b->pending) HIP_OK(hipEventSynchronize(b->done));
+            b->pending = false;
+            b->last.clear();  // the metadata goes with the buffers: the next launch uploads it again
+            b->meta.release();
+            b->ck.release();
+            b->yb.release();
+            b->part.release();
+        }
+        ctx->out_slot[0].release();
+        ctx->out_slot[1].release();
+        ctx->fwd_buf.release();
+    });
+}
+
 int gsmc_context_set_state_budget(gsmc_context* ctx, uint64_t max_state_bytes)
 {
     return guarded([&] {
@@ -1335,11 +1360,14 @@ int gsmc_run_fb_summary(gsmc_context* ctx, const int32_t* pi, const int32_t* pj,
         const int64_t c = core > 0 ? core : S;
         const int64_t fl = core > 0 ? flank : 0;
         // Pairs per chunk: partials (16 B per 64-pair group and site) and
-        // checkpoints (8 B per pair every K sites) within a quarter of free memory.
+        // checkpoints (8 B per pair every K sites) within half of the memory
+        // the launch can use: free memory plus the workspace buffers it will
+        // reuse (a workspace keeps its largest earlier launch's buffers).
         size_t free_b = 0, total_b = 0;
         HIP_OK(hipMemGetInfo(&free_b, &total_b));
+        const double avail = (double)free_b + (double)ctx->dev_batch->ck.bytes + (double)ctx->dev_batch->part.bytes;
         const double per_pair = 16.0 / 64.0 * (double)S + 8.0 * (double)(S + 2 * fl) / ctx->tune.K + 64.0;
-        int64_t chunk = (int64_t)(0.25 * (double)free_b / per_pair) / 128 * 128;
+        int64_t chunk = (int64_t)(0.5 * avail / per_pair) / 128 * 128;
         chunk = std::max<int64_t>(chunk, 128);
         chunk = std::min<int64_t>(chunk, (P + 127) / 128 * 128);
         ctx->sum_pairs.ensure((size_t)chunk * 2 * sizeof(int32_t), ctx->device);

@@ -149,6 +149,14 @@ int gsmc_context_set_tuning(gsmc_context* ctx, int checkpoint_k, int pairs_per_l
  * Results do not depend on it. No reference counterpart. */
 int gsmc_context_set_state_budget(gsmc_context* ctx, uint64_t max_state_bytes);
 
+/* Free the device workspaces the context keeps between calls (metadata,
+ * checkpoints / stored states, flank states, summary partials, output
+ * slots), after waiting for the context's work. A workspace keeps its
+ * largest launch's buffers; this returns that memory, for instance after a
+ * launch-shape sweep. The next call allocates what it needs again. No
+ * reference counterpart. */
+int gsmc_context_release_buffers(gsmc_context* ctx);
+
 /* Full-sequence forward-backward of n_pairs pairs (pair_i[p], pair_j[p]) into
  * host arrays laid out site-major [n_sites][n_pairs]: mean always; lower and
  * upper when flags & GSMC_CI; alpha and beta when flags & GSMC_POSTERIOR
