@@ -103,6 +103,23 @@ def test_blockwise_pair_chunks_and_streams_invariant(core, cohort_small, flow_fi
     assert _bits_equal(a["mean"], b["mean"])
 
 
+@pytest.mark.parametrize("n_pairs", [128, 130, 129, 64])
+def test_row_segment_stores_bit_exact(core, cohort_small, flow_field_path, n_pairs):
+    """Full 64-pair slots store their mean rows through an LDS tile as 16-byte
+    (P % 4 == 0) or 8-byte (P % 2 == 0) row segments, other slots lane by
+    lane; every pair count lands on the oracle's values bit for bit."""
+    G, pos = cohort_small
+    base = O.all_pairs(G.shape[0])
+    pairs = (base * 3)[:n_pairs]
+    T = _tables(flow_field_path, 1.25e-8, 1e-8)
+    ctx = core.FlowContext(G, pos, 1e4, 1.25e-8, 1e-8, flow_field_path, 0)
+    for K in (8, 16):
+        ctx.set_tuning(checkpoint_k=K)
+        _case(f"row stores P={n_pairs} K={K}")
+        got = ctx.run_fb_blockwise(pairs, core_block_sites=200, flank_sites=64, pair_batch_size=-1, max_streams=1)
+        assert _bits_equal(got["mean"], O.fb_blockwise_f32(G, pos, pairs, T, 1e4, 200, 64)["mean"]), K
+
+
 def test_gap_longer_than_cache(core, flow_field_path):
     # gaps beyond n_max_steps exercise the stretch-layer chunking
     rng = np.random.default_rng(5)
