@@ -39,7 +39,7 @@ HBM_PEAK_GBS = 8000.0
 # commands is profiles/fb_decode_chromosome_counters.md. Reported as
 # roofline.traffic only when a run matches that workload and shape.
 MEASURED_TRAFFIC = {
-    ("chromosome", 8, 1, "static", False): 186.96e9,
+    ("chromosome", 8, 1, "static", False): 197.77e9,
     ("chromosome", 16, 1, "static", False): 155.15e9,
 }
 
@@ -438,7 +438,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(the decoder shards over pairs with no communication; see DESIGN.md)"}))
+                                   "(the decoder would shard over pairs with no communication)"}))
         sys.exit(2)
 
     wl = WORKLOADS[args.workload]

@@ -558,6 +558,24 @@ __device__ __forceinline__ void store_codes(const FbArgs& a, const JobDev& j, co
     }
 }
 
+// The forward states of the window being walked backward, slot s of pair
+// q: in registers, or (LS) in LDS rows [NP][K][64] float2 of the wave,
+// which keeps 2 K NP registers out of the backward kernel's allocation.
+template <int K, int NP, bool LS> struct FwdStates {
+    float m[NP][K], c[NP][K];
+    __device__ __forceinline__ float2 get(int q, int s) const { return make_float2(m[q][s], c[q][s]); }
+    __device__ __forceinline__ void set(int q, int s, float vm, float vc)
+    {
+        m[q][s] = vm;
+        c[q][s] = vc;
+    }
+};
+template <int K, int NP> struct FwdStates<K, NP, true> {
+    float2* col;  // this lane's column of the rows
+    __device__ __forceinline__ float2 get(int q, int s) const { return col[(q * K + s) * 64]; }
+    __device__ __forceinline__ void set(int q, int s, float vm, float vc) { col[(q * K + s) * 64] = make_float2(vm, vc); }
+};
+
 // Phase-3 step over one checkpoint window, software-pipelined: the
 // backward walk of window w (cnt sites at ws, forward states in F) runs in
 // the same loop as the forward recompute of window w - 1 (K sites) when
@@ -569,11 +587,11 @@ __device__ __forceinline__ void store_codes(const FbArgs& a, const JobDev& j, co
 // store (vector-memory counters are shared by loads and stores). FAST (a
 // full window and a full next window, no long gaps in either) has no branch
 // inside the loop, so the compiler can interleave all chains.
-template <int K, int NP, bool CI, bool POST, bool SUM, bool B, bool FAST, bool ST>
+template <int K, int NP, bool CI, bool POST, bool SUM, bool B, bool FAST, bool ST, bool LS>
 __device__ __forceinline__ void window_step(const FbArgs& a, const JobDev& j, const Lanes<NP>& L, int64_t cs,
                                             int64_t ws, int64_t cnt, bool has_next, const Window<K, NP>& cur,
                                             const Window<K, NP>& nxt, float (&m)[NP], float (&c)[NP],
-                                            float (&fm)[NP][K], float (&fc)[NP][K], float (&ym)[NP],
+                                            FwdStates<K, NP, LS>& F, float (&ym)[NP],
                                             float (&yc)[NP], float* red)
 {
     float out[NP][K], oa[NP][K], ob[NP][K];
@@ -584,7 +602,8 @@ __device__ __forceinline__ void window_step(const FbArgs& a, const JobDev& j, co
         if (FAST || k < cnt) {
 #pragma unroll
             for (int q = 0; q < NP; ++q) {
-                out[q][k] = combine_mean(fm[q][slot], fc[q][slot], ym[q], yc[q], a.unscale, oa[q][k], ob[q][k]);
+                const float2 f = F.get(q, slot);
+                out[q][k] = combine_mean(f.x, f.y, ym[q], yc[q], a.unscale, oa[q][k], ob[q][k]);
                 if (FAST)
                     lookup(layer_of(a.t.bwd, cur.d[k]), het_slab(cur.bits[q], k), ym[q], yc[q]);
                 else
@@ -603,10 +622,8 @@ __device__ __forceinline__ void window_step(const FbArgs& a, const JobDev& j, co
                     if (i & 1) {
                         const f32x4 f = get_rec(rec_at(L, (ws - K + i - 1 - cs) >> 1, q));
                         const int prev = B ? i - 1 : K - i;
-                        fm[q][prev] = f.x;
-                        fc[q][prev] = f.y;
-                        fm[q][slot] = f.z;
-                        fc[q][slot] = f.w;
+                        F.set(q, prev, f.x, f.y);
+                        F.set(q, slot, f.z, f.w);
                     }
                     continue;
                 }
@@ -614,8 +631,7 @@ __device__ __forceinline__ void window_step(const FbArgs& a, const JobDev& j, co
                     lookup(layer_of(a.t.fwd, nxt.d[i]), het_slab(nxt.bits[q], i), m[q], c[q]);
                 else
                     fwd_segment(a.t, nxt.d[i], het_slab(nxt.bits[q], i), m[q], c[q]);
-                fm[q][slot] = m[q];  // next window: site ws - K + i, in the slot just freed
-                fc[q][slot] = c[q];
+                F.set(q, slot, m[q], c[q]);  // next window: site ws - K + i, in the slot just freed
             }
         }
         // Keep each site's work in its own scheduling region: the two chains
@@ -811,8 +827,9 @@ __device__ __forceinline__ void forward_item(const FbArgs& a, int item, int lane
 // recompute rides along the previous window's backward walk. ST: the
 // forward states are read back instead of recomputed, so the next
 // window's descriptors and het bits are not needed.
-template <int K, int NP, bool CI, bool POST, bool SUM, bool ST>
-__device__ __forceinline__ void backward_item(const FbArgs& a, int item, int lane, float* red, int slot = -1)
+template <int K, int NP, bool CI, bool POST, bool SUM, bool ST, bool LS = false>
+__device__ __forceinline__ void backward_item(const FbArgs& a, int item, int lane, float* red, int slot = -1,
+                                              float2* states = nullptr)
 {
     const ItemView<NP> v = open_item<NP>(a, item, lane, slot);
     if (!v.valid) return;
@@ -829,7 +846,8 @@ __device__ __forceinline__ void backward_item(const FbArgs& a, int item, int lan
         yc[q] = y.y;
         m[q] = c[q] = 0.0f;
     }
-    float fm[NP][K], fc[NP][K];
+    FwdStates<K, NP, LS> F;
+    if constexpr (LS) F.col = states + lane;
     int64_t w = (int64_t)b.nwin - 1;
     int64_t cnt = ce - (cs + w * K);
     HetCache<NP> hc, hn;  // current windows; next windows (recomputed forward states)
@@ -842,10 +860,8 @@ __device__ __forceinline__ void backward_item(const FbArgs& a, int item, int lan
 #pragma unroll
                     for (int q = 0; q < NP; ++q) {
                         const f32x4 f = get_rec(rec_at(L, (ws + k - cs) >> 1, q));
-                        fm[q][k] = f.x;
-                        fc[q][k] = f.y;
-                        fm[q][k + 1] = f.z;
-                        fc[q][k + 1] = f.w;
+                        F.set(q, k, f.x, f.y);
+                        F.set(q, k + 1, f.z, f.w);
                     }
                 }
             }
@@ -865,8 +881,7 @@ __device__ __forceinline__ void backward_item(const FbArgs& a, int item, int lan
 #pragma unroll
                     for (int q = 0; q < NP; ++q) {
                         fwd_segment(a.t, win.d[k], het_slab(win.bits[q], k), m[q], c[q]);
-                        fm[q][k] = m[q];
-                        fc[q][k] = c[q];
+                        F.set(q, k, m[q], c[q]);
                     }
                 }
             }
@@ -897,18 +912,18 @@ __device__ __forceinline__ void backward_item(const FbArgs& a, int item, int lan
         const bool fast = has_next && cnt == K && no_long_gaps<K>(cur.d) && (ST || no_long_gaps<K>(nxt.d));
         if (fast) {
             if (orient_b)
-                window_step<K, NP, CI, POST, SUM, true, true, ST>(a, j, L, cs, ws, cnt, true, cur, nxt, m, c, fm, fc, ym,
-                                                                  yc, red);
+                window_step<K, NP, CI, POST, SUM, true, true, ST, LS>(a, j, L, cs, ws, cnt, true, cur, nxt, m, c, F, ym,
+                                                                      yc, red);
             else
-                window_step<K, NP, CI, POST, SUM, false, true, ST>(a, j, L, cs, ws, cnt, true, cur, nxt, m, c, fm, fc, ym,
-                                                                   yc, red);
+                window_step<K, NP, CI, POST, SUM, false, true, ST, LS>(a, j, L, cs, ws, cnt, true, cur, nxt, m, c, F, ym,
+                                                                       yc, red);
         } else {
             if (orient_b)
-                window_step<K, NP, CI, POST, SUM, true, false, ST>(a, j, L, cs, ws, cnt, has_next, cur, nxt, m, c, fm, fc,
-                                                                   ym, yc, red);
+                window_step<K, NP, CI, POST, SUM, true, false, ST, LS>(a, j, L, cs, ws, cnt, has_next, cur, nxt, m, c, F,
+                                                                       ym, yc, red);
             else
-                window_step<K, NP, CI, POST, SUM, false, false, ST>(a, j, L, cs, ws, cnt, has_next, cur, nxt, m, c, fm,
-                                                                    fc, ym, yc, red);
+                window_step<K, NP, CI, POST, SUM, false, false, ST, LS>(a, j, L, cs, ws, cnt, has_next, cur, nxt, m, c, F,
+                                                                        ym, yc, red);
         }
         if (!has_next) break;
         orient_b = !orient_b;
@@ -917,10 +932,20 @@ __device__ __forceinline__ void backward_item(const FbArgs& a, int item, int lan
     }
 }
 
-// Waves per SIMD the backward kernel is built for: three with one pair per
-// lane and recomputed forward states (a few registers over would leave
-// two), no floor otherwise.
-template <int NP, bool ST> constexpr int kBwdWaves = (NP == 1 && !ST) ? 3 : 1;
+
+// Forward states of the backward kernels: in LDS for K = 8 (recomputed
+// states), which takes the K = 8 mean kernel from 123 to 77 registers, so
+// six waves per SIMD instead of four hide the latency of each lookup's
+// gathers; in registers for K = 16 (its 8 KB of states per wave would cap
+// the waves per CU) and for stored states.
+template <int K, bool ST> constexpr bool kStatesInLds = !ST && K == 8;
+template <int K, int NP, bool ST> constexpr int kStateLds = kStatesInLds<K, ST> ? K * NP * 64 : 1;
+
+// Waves per SIMD the backward kernel is built for: six with one pair per
+// lane and forward states in LDS, three with them in registers (a few
+// registers over would leave two), no floor otherwise.
+template <int K, int NP, bool ST>
+constexpr int kBwdWaves = (NP == 1 && kStatesInLds<K, ST>) ? 5 : (NP == 1 && !ST) ? 3 : 1;
 
 // LDS floats of the backward kernel: the summary's staging rows, or the
 // [K][64] tile of float-mean row stores (reused slot by slot).
@@ -941,11 +966,13 @@ __global__ __launch_bounds__(64) void fb_fwd_items(FbArgs a)
 }
 
 template <int K, int NP, bool CI, bool POST, bool SUM, bool ST>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kBwdWaves<NP, ST>))) void fb_bwd_items(FbArgs a)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kBwdWaves<K, NP, ST>))) void fb_bwd_items(FbArgs a)
 {
     const int item = __builtin_amdgcn_readfirstlane((int)blockIdx.x);
     __shared__ float red[kBwdLds<K, CI, POST, SUM>];
-    if (item < a.n_items) backward_item<K, NP, CI, POST, SUM, ST>(a, item, threadIdx.x, red);
+    __shared__ float2 states[kStateLds<K, NP, ST>];
+    if (item < a.n_items)
+        backward_item<K, NP, CI, POST, SUM, ST, kStatesInLds<K, ST>>(a, item, threadIdx.x, red, -1, states);
 }
 
 __device__ __forceinline__ int next_item(int* counter, int lane)
@@ -963,11 +990,12 @@ __global__ __launch_bounds__(64) void fb_fwd_queue(FbArgs a)
 }
 
 template <int K, int NP, bool CI, bool POST, bool SUM, bool ST>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kBwdWaves<NP, ST>))) void fb_bwd_queue(FbArgs a)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kBwdWaves<K, NP, ST>))) void fb_bwd_queue(FbArgs a)
 {
     __shared__ float red[kBwdLds<K, CI, POST, SUM>];
+    __shared__ float2 states[kStateLds<K, NP, ST>];
     for (int item = next_item(a.queue2, threadIdx.x); item < a.n_items; item = next_item(a.queue2, threadIdx.x))
-        backward_item<K, NP, CI, POST, SUM, ST>(a, item, threadIdx.x, red);
+        backward_item<K, NP, CI, POST, SUM, ST, kStatesInLds<K, ST>>(a, item, threadIdx.x, red, -1, states);
 }
 
 // Stored forward states, one launch: persistent waves take items from the
