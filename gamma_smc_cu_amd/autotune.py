@@ -7,9 +7,10 @@ fastest. Candidates sweep:
 
 * ``checkpoint_k`` -- sites per checkpoint window, which is also the length
   of the forward window recomputed into registers (8 or 16), or 1: every
-  forward state stored by the forward kernel and read back by the backward
-  kernel, one lookup per site fewer for 16 bytes of HBM traffic per
-  pair-site;
+  forward state stored by the forward walk and read back by the backward
+  walk, one lookup per site fewer for 16 bytes of HBM traffic per
+  pair-site (two kernels per phase of items with the static schedule, one
+  fused kernel with the persistent one);
 * ``pairs_per_lane`` -- 1 or 2 pairs per lane, i.e. 64 or 128 pairs per
   64-lane workgroup (two pairs give every phase two independent lookup
   chains per lane, at the cost of registers);

@@ -490,9 +490,11 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // Forward states stored at every core site (instead of checkpoints every K
 // sites and a recompute): tuning K = 1, for the mean / code / summary
-// kernels. Those run as one fused persistent kernel whose forward half
-// walks windows of 16 sites and backward half windows of kStoredBwdK (short
-// windows keep the kernel at six waves per SIMD).
+// kernels. The forward walk goes in windows of 16 sites and the backward
+// walk in windows of kStoredBwdK (short windows keep it at six waves per
+// SIMD): as two kernels per phase of items (static schedule), or as one
+// fused persistent kernel with a state slot per resident wave (persistent
+// schedule).
 bool stored_states(const Tuning& tu, int flags) { return tu.K == 1 && !(flags & (GSMC_CI | GSMC_POSTERIOR)); }
 int window_k(const Tuning& tu) { return tu.K == 8 ? 8 : 16; }
 constexpr int kStoredBwdK = 4;
@@ -595,7 +597,7 @@ Plan make_plan(const gsmc_context* ctx, const std::vector<JobSpec>& jobs, const 
         };
         return cost(x) > cost(y);
     });
-    if (stored) {
+    if (stored && tu.schedule == 1) {
         // one fused launch: a state slot (the largest item's rows) per
         // resident wave, as many waves as fit in the budget
         size_t slot = 0;
@@ -725,6 +727,17 @@ template <int K> KernelPair kernels_k(const Tuning& tu, int flags)
 
 KernelPair kernels_for(const Tuning& tu, int flags)
 {
+    if (stored_states(tu, flags)) {
+        if (flags & GSMC_SUMMARY)
+            return {reinterpret_cast<const void*>(&fb_fwd_items<16, 1, true>),
+                    reinterpret_cast<const void*>(&fb_bwd_items<kStoredBwdK, 1, false, false, true, true>),
+                    reinterpret_cast<const void*>(&fb_fwd_queue<16, 1, true>),
+                    reinterpret_cast<const void*>(&fb_bwd_queue<kStoredBwdK, 1, false, false, true, true>)};
+        return {reinterpret_cast<const void*>(&fb_fwd_items<16, 1, true>),
+                reinterpret_cast<const void*>(&fb_bwd_items<kStoredBwdK, 1, false, false, false, true>),
+                reinterpret_cast<const void*>(&fb_fwd_queue<16, 1, true>),
+                reinterpret_cast<const void*>(&fb_bwd_queue<kStoredBwdK, 1, false, false, false, true>)};
+    }
     return window_k(tu) == 8 ? kernels_k<8>(tu, flags) : kernels_k<16>(tu, flags);
 }
 

@@ -130,9 +130,11 @@ int gsmc_context_info(const gsmc_context* ctx, int64_t* n_haps, int64_t* n_sites
 /* Decoder tuning; results do not depend on any of it. checkpoint_k: sites
  * per checkpoint window, 8 or 16 (default 16), or 1: the forward kernel
  * stores its state at every core site and the backward kernel reads it
- * back instead of recomputing it (8 B per pair and core site of device
- * memory, launches split into phases that fit; mean, code and summary
- * launches, CI / posterior launches use 16); pairs_per_lane: 1 or 2
+ * back instead of recomputing it (mean, code and summary launches; CI /
+ * posterior launches use 16). With the static schedule that takes 8 B per
+ * pair and core site of device memory, in launches split into phases that
+ * fit; with the persistent schedule one fused kernel reuses a state slot
+ * per resident wave. pairs_per_lane: 1 or 2
  * (default 1), i.e. 64 or 128 pairs per 64-lane workgroup, for the
  * mean-only kernel (CI / posterior launches use 1); schedule: 0 static grid
  * of work items, 1 persistent waves draining a device queue (default 0);
