@@ -1542,11 +1542,14 @@ int gsmc_batch_reserve(gsmc_batch* b, uint64_t checkpoint_bytes, int64_t max_ite
     });
 }
 
+// C++ linkage for helpers that return C++ types, inside the C ABI block
+extern "C++" {
 namespace gsmc {
 namespace {
 std::vector<JobSpec> specs_of(const gsmc_context* ctx, const gsmc_job* jobs, int64_t n_jobs, int flags);
 }
 }  // namespace gsmc
+}  // extern "C++"
 
 // A captured launch: private buffers with the plan's metadata uploaded, and
 // an executable graph of the launch's kernels.
@@ -1614,6 +1617,7 @@ int gsmc_graph_launch(gsmc_graph* g, void* stream)
 
 void gsmc_graph_destroy(gsmc_graph* g) { delete g; }
 
+extern "C++" {
 namespace gsmc {
 namespace {
 std::vector<JobSpec> specs_of(const gsmc_context* ctx, const gsmc_job* jobs, int64_t n_jobs, int flags)
@@ -1653,6 +1657,7 @@ std::vector<JobSpec> specs_of(const gsmc_context* ctx, const gsmc_job* jobs, int
 }
 }  // namespace
 }  // namespace gsmc
+}  // extern "C++"
 
 int gsmc_batch_launch(gsmc_batch* b, const gsmc_job* jobs, int64_t n_jobs, int flags, void* stream)
 {
