@@ -394,6 +394,23 @@ __device__ __forceinline__ float2* ck_at(const Lanes<NP>& L, int64_t w, int q)
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
+// Checkpoints are written once by the forward kernel and read once by the
+// backward kernel after every other item's forward walk, so no cache level
+// holds them until then: they stream past L2 (non-temporal), as the
+// outputs do, and leave it to the transition tables.
+__device__ __forceinline__ void put_ck(float2* p, float m, float c)
+{
+    f32x2 v;
+    v.x = m;
+    v.y = c;
+    __builtin_nontemporal_store(v, reinterpret_cast<f32x2*>(p));
+}
+__device__ __forceinline__ float2 get_ck(const float2* p)
+{
+    const f32x2 v = __builtin_nontemporal_load(reinterpret_cast<const f32x2*>(p));
+    return make_float2(v.x, v.y);
+}
+
 // Stored forward states: one 16-byte record per lane and pair of core
 // sites (cs + 2t, cs + 2t + 1), records t of the item's 64 NP lanes in row
 // t. The address path spends the same cycles on a 16-byte access as on an
@@ -751,7 +768,7 @@ __device__ __forceinline__ void forward_item(const FbArgs& a, int item, int lane
         const bool core = s >= cs;  // windows are all flank or all core
         if (!ST && core) {
 #pragma unroll
-            for (int q = 0; q < NP; ++q) *ck_at(L, (s - cs) / K, q) = make_float2(m[q], c[q]);
+            for (int q = 0; q < NP; ++q) put_ck(ck_at(L, (s - cs) / K, q), m[q], c[q]);
         }
         Window<K, NP> win;
         load_window(a, L, s, win, hf);
@@ -871,7 +888,7 @@ __device__ __forceinline__ void backward_item(const FbArgs& a, int item, int lan
             if (ws == ps) win.d[0] = b.sd_fwd;
 #pragma unroll
             for (int q = 0; q < NP; ++q) {
-                const float2 st = *ck_at(L, w, q);
+                const float2 st = get_ck(ck_at(L, w, q));
                 m[q] = st.x;
                 c[q] = st.y;
             }
@@ -899,7 +916,7 @@ __device__ __forceinline__ void backward_item(const FbArgs& a, int item, int lan
             if (ws - K == ps) nxt.d[0] = b.sd_fwd;
 #pragma unroll
             for (int q = 0; q < NP; ++q) {
-                const float2 st = *ck_at(L, w - 1, q);
+                const float2 st = get_ck(ck_at(L, w - 1, q));
                 m[q] = st.x;
                 c[q] = st.y;
             }
