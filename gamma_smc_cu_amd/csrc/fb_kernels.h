@@ -1177,14 +1177,20 @@ __global__ __launch_bounds__(64) void fwd_only_kernel(FwdArgs a)
                     lookup_h2(a.h2q + (size_t)(dk & kDescLayerMask) * kQuadSlabEntries, m, c);
                 }
                 if (het_bit(bits, k)) {
-                    // the reference's fast 10^x and log10 (gamma_smc_flow.cu:1234-1235)
+                    // fast 10^x and log10 where the reference uses its fast
+                    // intrinsics (gamma_smc_flow.cu:1234-1235): log10 as the
+                    // hardware log2 times log10(2) (alpha >= 1, so no
+                    // denormal range to scale for)
                     const float alpha = __exp10f(-2.0f * c) + 1.0f;
-                    const float a_log = __log10f(alpha);
+                    const float a_log = __builtin_amdgcn_logf(alpha) * 0.30102999566398120f;
                     const float b_log = fmaf(-2.0f, c, -m);
                     m = a_log - b_log;
                     c = -0.5f * a_log;
                 }
-                const float mg = exp2f(m * 3.321928094887362f) * a.two_ne;
+                // 10^m by the hardware 2^x: table states lie in the grid box
+                // (m in [-5, 2]) and an emission raises m by at most log10(2),
+                // far from the range ends that need scaling
+                const float mg = __builtin_amdgcn_exp2f(m * 3.321928094887362f) * a.two_ne;
                 out[k] = mg;
                 if (CI) {
                     const float alpha = __exp10f(-2.0f * c);
