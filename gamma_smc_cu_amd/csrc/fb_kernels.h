@@ -1133,7 +1133,8 @@ __device__ __forceinline__ void lookup_h2(const uint4* __restrict__ layer, float
     fc = __builtin_amdgcn_fmed3f(fc, 0.0f, 49.0f);
     const float lm = fminf(floorf(fm), 49.0f), lc = fminf(floorf(fc), 48.0f);
     const float wm = fm - lm, wc = fc - lc;
-    const uint4 e = layer[(uint32_t)fmaf(lm, 50.0f, lc)];
+    // 32-bit byte offset from the layer base (scalar base + vector offset)
+    const uint4 e = *global_at<const uint4>((void*)layer, (uint32_t)fmaf(lm, 50.0f, lc) << 4);
     const float2 v00 = half2_bits(e.x), v10 = half2_bits(e.y), v01 = half2_bits(e.z), v11 = half2_bits(e.w);
     const float omwm = 1.0f - wm, omwc = 1.0f - wc;
     m = fmaf(wm, fmaf(wc, v11.x, omwc * v10.x), omwm * fmaf(wc, v01.x, omwc * v00.x));
@@ -1166,55 +1167,65 @@ __global__ __launch_bounds__(64) void fwd_only_kernel(FwdArgs a)
         }
         const int sh = (int)(s0 & 63);
         const uint64_t bits = sh == 0 ? h.lo[0] : (h.lo[0] >> sh) | (h.hi[0] << (64 - sh));
-        const int64_t cnt = min((int64_t)16, a.S - s0);
+        const int cnt = (int)min((int64_t)16, a.S - s0);
         float out[16], olo[16], ohi[16];
-#pragma unroll
-        for (int k = 0; k < 16; ++k) {
-            if (k < cnt) {
-                const uint32_t dk = d[k];
-                if (dk != kNoAdvance) {
-                    for (uint32_t q = dk >> kDescLayerBits; q != 0; --q) lookup_h2(last, m, c);
-                    lookup_h2(a.h2q + (size_t)(dk & kDescLayerMask) * kQuadSlabEntries, m, c);
-                }
-                if (het_bit(bits, k)) {
-                    // fast 10^x and log10 where the reference uses its fast
-                    // intrinsics (gamma_smc_flow.cu:1234-1235): log10 as the
-                    // hardware log2 times log10(2) (alpha >= 1, so no
-                    // denormal range to scale for)
-                    const float alpha = __exp10f(-2.0f * c) + 1.0f;
-                    const float a_log = __builtin_amdgcn_logf(alpha) * 0.30102999566398120f;
-                    const float b_log = fmaf(-2.0f, c, -m);
-                    m = a_log - b_log;
-                    c = -0.5f * a_log;
-                }
-                // 10^m by the hardware 2^x: table states lie in the grid box
-                // (m in [-5, 2]) and an emission raises m by at most log10(2),
-                // far from the range ends that need scaling
-                const float mg = __builtin_amdgcn_exp2f(m * 3.321928094887362f) * a.two_ne;
-                out[k] = mg;
-                if (CI) {
-                    const float alpha = __exp10f(-2.0f * c);
-                    const float inv9a = 1.0f / (9.0f * alpha);
-                    const float sq = sqrtf(inv9a);
-                    const float base = 1.0f - inv9a;
-                    const float lo = fmaxf(fmaf(-1.96f, sq, base), 0.0f);
-                    const float hf = fmaf(1.96f, sq, base);
-                    olo[k] = fmaxf(mg * lo * lo * lo, 0.0f);
-                    ohi[k] = mg * hf * hf * hf;
-                }
+        auto site = [&](int k) {
+            const uint32_t dk = d[k];
+            if (dk != kNoAdvance) {
+                for (uint32_t q = dk >> kDescLayerBits; q != 0; --q) lookup_h2(last, m, c);
+                lookup_h2(a.h2q + (size_t)(dk & kDescLayerMask) * kQuadSlabEntries, m, c);
             }
-        }
-        if (live) {
+            if (het_bit(bits, k)) {
+                // fast 10^x and log10 where the reference uses its fast
+                // intrinsics (gamma_smc_flow.cu:1234-1235): log10 as the
+                // hardware log2 times log10(2) (alpha >= 1, so no
+                // denormal range to scale for)
+                const float alpha = __exp10f(-2.0f * c) + 1.0f;
+                const float a_log = __builtin_amdgcn_logf(alpha) * 0.30102999566398120f;
+                const float b_log = fmaf(-2.0f, c, -m);
+                m = a_log - b_log;
+                c = -0.5f * a_log;
+            }
+            // 10^m by the hardware 2^x: table states lie in the grid box
+            // (m in [-5, 2]) and an emission raises m by at most log10(2),
+            // far from the range ends that need scaling
+            const float mg = __builtin_amdgcn_exp2f(m * 3.321928094887362f) * a.two_ne;
+            out[k] = mg;
+            if (CI) {
+                const float alpha = __exp10f(-2.0f * c);
+                const float inv9a = 1.0f / (9.0f * alpha);
+                const float sq = sqrtf(inv9a);
+                const float base = 1.0f - inv9a;
+                const float lo = fmaxf(fmaf(-1.96f, sq, base), 0.0f);
+                const float hf = fmaf(1.96f, sq, base);
+                olo[k] = fmaxf(mg * lo * lo * lo, 0.0f);
+                ohi[k] = mg * hf * hf * hf;
+            }
+        };
+        auto store = [&](int k) {
+            const size_t idx = (size_t)(s0 + k) * (size_t)a.P + (size_t)p;
+            __builtin_nontemporal_store(out[k], a.mean + idx);
+            if (CI) {
+                __builtin_nontemporal_store(olo[k], a.lower + idx);
+                __builtin_nontemporal_store(ohi[k], a.upper + idx);
+            }
+        };
+        // full chunks run without a bounds test per site
+        if (cnt == 16) {
 #pragma unroll
-            for (int k = 0; k < 16; ++k) {
-                if (k < cnt) {
-                    const size_t idx = (size_t)(s0 + k) * (size_t)a.P + (size_t)p;
-                    __builtin_nontemporal_store(out[k], a.mean + idx);
-                    if (CI) {
-                        __builtin_nontemporal_store(olo[k], a.lower + idx);
-                        __builtin_nontemporal_store(ohi[k], a.upper + idx);
-                    }
-                }
+            for (int k = 0; k < 16; ++k) site(k);
+            if (live) {
+#pragma unroll
+                for (int k = 0; k < 16; ++k) store(k);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 16; ++k)
+                if (k < cnt) site(k);
+            if (live) {
+#pragma unroll
+                for (int k = 0; k < 16; ++k)
+                    if (k < cnt) store(k);
             }
         }
     }
