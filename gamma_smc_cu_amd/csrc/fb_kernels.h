@@ -24,6 +24,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace gsmc {
 
 constexpr int kGridNodes = 2550;
@@ -1169,22 +1171,29 @@ __global__ __launch_bounds__(64) void fwd_only_kernel(FwdArgs a)
         const uint64_t bits = sh == 0 ? h.lo[0] : (h.lo[0] >> sh) | (h.hi[0] << (64 - sh));
         const int cnt = (int)min((int64_t)16, a.S - s0);
         float out[16], olo[16], ohi[16];
-        auto site = [&](int k) {
+        // PLAIN: the site has one transition of at most N bp (no stretch
+        // chunks, not a site without a transition)
+        auto site = [&](int k, auto plain) {
             const uint32_t dk = d[k];
-            if (dk != kNoAdvance) {
+            if (decltype(plain)::value) {
+                lookup_h2(a.h2q + (size_t)dk * kQuadSlabEntries, m, c);
+            } else if (dk != kNoAdvance) {
                 for (uint32_t q = dk >> kDescLayerBits; q != 0; --q) lookup_h2(last, m, c);
                 lookup_h2(a.h2q + (size_t)(dk & kDescLayerMask) * kQuadSlabEntries, m, c);
             }
-            if (het_bit(bits, k)) {
-                // fast 10^x and log10 where the reference uses its fast
-                // intrinsics (gamma_smc_flow.cu:1234-1235): log10 as the
-                // hardware log2 times log10(2) (alpha >= 1, so no
-                // denormal range to scale for)
+            {
+                // het emission, fast 10^x and log10 where the reference uses
+                // its fast intrinsics (gamma_smc_flow.cu:1234-1235): log10 as
+                // the hardware log2 times log10(2) (alpha >= 1, so no
+                // denormal range to scale for). Some lane of the wave is het
+                // at almost every site, so every lane computes it and the
+                // het lanes take it (no divergent branch per site).
+                const bool het = het_bit(bits, k);
                 const float alpha = __exp10f(-2.0f * c) + 1.0f;
                 const float a_log = __builtin_amdgcn_logf(alpha) * 0.30102999566398120f;
                 const float b_log = fmaf(-2.0f, c, -m);
-                m = a_log - b_log;
-                c = -0.5f * a_log;
+                m = het ? a_log - b_log : m;
+                c = het ? -0.5f * a_log : c;
             }
             // 10^m by the hardware 2^x: table states lie in the grid box
             // (m in [-5, 2]) and an emission raises m by at most log10(2),
@@ -1210,10 +1219,22 @@ __global__ __launch_bounds__(64) void fwd_only_kernel(FwdArgs a)
                 __builtin_nontemporal_store(ohi[k], a.upper + idx);
             }
         };
-        // full chunks run without a bounds test per site
-        if (cnt == 16) {
+        // Full chunks run without a bounds test per site, and, when every
+        // site is plain (the usual case; the no-transition marker has its
+        // high bits set too), without a descriptor test per site.
+        uint32_t any_high = 0;
 #pragma unroll
-            for (int k = 0; k < 16; ++k) site(k);
+        for (int k = 0; k < 16; ++k) any_high |= d[k];
+        if (cnt == 16 && (any_high >> kDescLayerBits) == 0) {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) site(k, std::true_type{});
+            if (live) {
+#pragma unroll
+                for (int k = 0; k < 16; ++k) store(k);
+            }
+        } else if (cnt == 16) {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) site(k, std::false_type{});
             if (live) {
 #pragma unroll
                 for (int k = 0; k < 16; ++k) store(k);
@@ -1221,7 +1242,7 @@ __global__ __launch_bounds__(64) void fwd_only_kernel(FwdArgs a)
         } else {
 #pragma unroll
             for (int k = 0; k < 16; ++k)
-                if (k < cnt) site(k);
+                if (k < cnt) site(k, std::false_type{});
             if (live) {
 #pragma unroll
                 for (int k = 0; k < 16; ++k)
