@@ -1189,11 +1189,14 @@ __global__ __launch_bounds__(64) void fwd_only_kernel(FwdArgs a)
                 // at almost every site, so every lane computes it and the
                 // het lanes take it (no divergent branch per site).
                 const bool het = het_bit(bits, k);
-                const float alpha = __exp10f(-2.0f * c) + 1.0f;
-                const float a_log = __builtin_amdgcn_logf(alpha) * 0.30102999566398120f;
+                // 10^(-2c) as 2^(c * -2 log2(10)); -log10(alpha) / 2 with the
+                // halving folded into the constant (both exact rescalings)
+                const float alpha = __builtin_amdgcn_exp2f(c * -6.64385618977472470f) + 1.0f;
+                const float l2 = __builtin_amdgcn_logf(alpha);
+                const float a_log = l2 * 0.30102999566398120f;
                 const float b_log = fmaf(-2.0f, c, -m);
                 m = het ? a_log - b_log : m;
-                c = het ? -0.5f * a_log : c;
+                c = het ? l2 * -0.15051499783199060f : c;
             }
             // 10^m by the hardware 2^x: table states lie in the grid box
             // (m in [-5, 2]) and an emission raises m by at most log10(2),
