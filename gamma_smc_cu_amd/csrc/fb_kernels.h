@@ -482,8 +482,9 @@ __device__ __forceinline__ void summarize(const FbArgs& a, const JobDev& j, cons
                                           int64_t cnt, const float (&out)[NP][K], float* red)
 {
     const int lane = (int)threadIdx.x;
-    // lanes 16 p .. 16 p + 15 read part p of rows 0..15: within each 32-lane
-    // half of a read, rows hit banks k and parts banks 16 p, all distinct
+    // lanes 16 p .. 16 p + 15 read part p of rows 0..K-1 (k < K): within
+    // each 32-lane half of a read, rows hit banks k and parts banks 16 p,
+    // all distinct
     const int k = lane & 15, part = lane >> 4;
 #pragma unroll
     for (int q = 0; q < NP; ++q) {
@@ -966,10 +967,11 @@ template <int K, int NP, bool ST> constexpr int kStateLds = kStatesInLds<K, ST> 
 template <int K, int NP, bool ST>
 constexpr int kBwdWaves = (NP == 1 && kStatesInLds<K, ST>) ? 5 : (NP == 1 && !ST) ? 3 : 1;
 
-// LDS floats of the backward kernel: the summary's staging rows, or the
-// [K][64] tile of float-mean row stores (reused slot by slot).
+// LDS floats of the backward kernel: the summary's staging rows (one row
+// of 65 per site of the window), or the [K][64] tile of float-mean row
+// stores (reused slot by slot).
 template <int K, bool CI, bool POST, bool SUM>
-constexpr int kBwdLds = SUM ? 16 * 65 : (CI || POST) ? 1 : K * 64;
+constexpr int kBwdLds = SUM ? K * 65 : (CI || POST) ? 1 : K * 64;
 
 // A decode is two launches over the same work items: fb_fwd_* (forward
 // walk + right-flank backward walk; few registers, so many waves per SIMD
