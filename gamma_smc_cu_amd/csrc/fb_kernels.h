@@ -961,9 +961,11 @@ __device__ __forceinline__ void backward_item(const FbArgs& a, int item, int lan
 template <int K, bool ST> constexpr bool kStatesInLds = !ST && K == 8;
 template <int K, int NP, bool ST> constexpr int kStateLds = kStatesInLds<K, ST> ? K * NP * 64 : 1;
 
-// Waves per SIMD the backward kernel is built for: six with one pair per
-// lane and forward states in LDS, three with them in registers (a few
-// registers over would leave two), no floor otherwise.
+// Waves per SIMD the backward kernel is built for (a floor for the register
+// allocator): five with one pair per lane and forward states in LDS (the
+// mean kernel needs 77 registers and gets six; the persistent and CI /
+// posterior kernels are held to 96 registers, five waves), three with them
+// in registers (a few registers over would leave two), no floor otherwise.
 template <int K, int NP, bool ST>
 constexpr int kBwdWaves = (NP == 1 && kStatesInLds<K, ST>) ? 5 : (NP == 1 && !ST) ? 3 : 1;
 
