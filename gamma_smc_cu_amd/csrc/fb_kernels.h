@@ -387,37 +387,18 @@ __device__ __forceinline__ void load_window(const FbArgs& a, const Lanes<NP>& L,
     for (int q = 0; q < NP; ++q) win.bits[q] = sh == 0 ? h.lo[q] : (h.lo[q] >> sh) | (h.hi[q] << (64 - sh));
 }
 
-template <int NP>
-__device__ __forceinline__ float2* ck_at(const Lanes<NP>& L, int64_t w, int q)
-{
-    return global_at<float2>(L.ck + (size_t)w * 64 * NP, L.ck_col + (uint32_t)q * 512u);
-}
-
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// Checkpoints are written once by the forward kernel and read once by the
-// backward kernel after every other item's forward walk, so no cache level
-// holds them until then: they stream past L2 (non-temporal), as the
-// outputs do, and leave it to the transition tables.
-__device__ __forceinline__ void put_ck(float2* p, float m, float c)
-{
-    f32x2 v;
-    v.x = m;
-    v.y = c;
-    __builtin_nontemporal_store(v, reinterpret_cast<f32x2*>(p));
-}
-__device__ __forceinline__ float2 get_ck(const float2* p)
-{
-    const f32x2 v = __builtin_nontemporal_load(reinterpret_cast<const f32x2*>(p));
-    return make_float2(v.x, v.y);
-}
-
-// Stored forward states: one 16-byte record per lane and pair of core
-// sites (cs + 2t, cs + 2t + 1), records t of the item's 64 NP lanes in row
-// t. The address path spends the same cycles on a 16-byte access as on an
-// 8-byte one, so pairing sites halves its share of the states. The records
-// go out and come back once each, a kernel apart: streaming accesses that
+// State records: one 16-byte record per lane holds two states, records t
+// of the item's 64 NP lanes in row t -- stored forward states of core
+// sites (cs + 2t, cs + 2t + 1), or the checkpoints of windows 2t and
+// 2t + 1. The address path spends the same cycles on a 16-byte access as
+// on an 8-byte one, so pairing halves its share; and since the vector
+// memory counter retires loads in order, each record load (an HBM round
+// trip) holds up the table gathers issued after it, so one load per two
+// checkpoint windows halves those stalls too. The records go out and come
+// back once each, a kernel apart: streaming (non-temporal) accesses that
 // leave the caches to the transition tables.
 template <int NP> __device__ __forceinline__ f32x4* rec_at(const Lanes<NP>& L, int64_t t, int q)
 {
@@ -766,12 +747,25 @@ __device__ __forceinline__ void forward_item(const FbArgs& a, int item, int lane
         c[q] = x0.y;
     }
     HetCache<NP> hf, hb;
+    float ckm[NP], ckc[NP];  // the even window's checkpoint until its odd partner's
     for (int64_t s = ps; s < ce;) {
         const int64_t e = (s < cs) ? min(cs, s + K) : min(ce, s + K);
         const bool core = s >= cs;  // windows are all flank or all core
         if (!ST && core) {
+            // checkpoints of windows 2r and 2r + 1 share 16-byte record r; a
+            // last even window fills both halves
+            const int64_t w = (s - cs) / K;
 #pragma unroll
-            for (int q = 0; q < NP; ++q) put_ck(ck_at(L, (s - cs) / K, q), m[q], c[q]);
+            for (int q = 0; q < NP; ++q) {
+                if (w & 1) {
+                    put_rec(rec_at(L, w >> 1, q), ckm[q], ckc[q], m[q], c[q]);
+                } else if (w + 1 == b.nwin) {
+                    put_rec(rec_at(L, w >> 1, q), m[q], c[q], m[q], c[q]);
+                } else {
+                    ckm[q] = m[q];
+                    ckc[q] = c[q];
+                }
+            }
         }
         Window<K, NP> win;
         load_window(a, L, s, win, hf);
@@ -868,6 +862,7 @@ __device__ __forceinline__ void backward_item(const FbArgs& a, int item, int lan
     }
     FwdStates<K, NP, LS> F;
     if constexpr (LS) F.col = states + lane;
+    float ckm[NP], ckc[NP];  // a loaded record's other checkpoint (window w - 1 when w is odd)
     int64_t w = (int64_t)b.nwin - 1;
     int64_t cnt = ce - (cs + w * K);
     HetCache<NP> hc, hn;  // current windows; next windows (recomputed forward states)
@@ -891,9 +886,12 @@ __device__ __forceinline__ void backward_item(const FbArgs& a, int item, int lan
             if (ws == ps) win.d[0] = b.sd_fwd;
 #pragma unroll
             for (int q = 0; q < NP; ++q) {
-                const float2 st = get_ck(ck_at(L, w, q));
-                m[q] = st.x;
-                c[q] = st.y;
+                // record w >> 1 holds the checkpoints of windows w & ~1 and w | 1
+                const f32x4 r = get_rec(rec_at(L, w >> 1, q));
+                m[q] = (w & 1) ? r.z : r.x;
+                c[q] = (w & 1) ? r.w : r.y;
+                ckm[q] = r.x;  // window w - 1's checkpoint when w is odd
+                ckc[q] = r.y;
             }
 #pragma unroll
             for (int k = 0; k < K; ++k) {
@@ -919,9 +917,17 @@ __device__ __forceinline__ void backward_item(const FbArgs& a, int item, int lan
             if (ws - K == ps) nxt.d[0] = b.sd_fwd;
 #pragma unroll
             for (int q = 0; q < NP; ++q) {
-                const float2 st = get_ck(ck_at(L, w - 1, q));
-                m[q] = st.x;
-                c[q] = st.y;
+                if ((w - 1) & 1) {
+                    // one load for windows w - 1 (now) and w - 2 (next window)
+                    const f32x4 r = get_rec(rec_at(L, (w - 1) >> 1, q));
+                    m[q] = r.z;
+                    c[q] = r.w;
+                    ckm[q] = r.x;
+                    ckc[q] = r.y;
+                } else {
+                    m[q] = ckm[q];
+                    c[q] = ckc[q];
+                }
             }
         } else {
 #pragma unroll

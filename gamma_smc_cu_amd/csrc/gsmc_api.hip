@@ -576,8 +576,9 @@ Plan make_plan(const gsmc_context* ctx, const std::vector<JobSpec>& jobs, const 
             d.job = (int32_t)(jd.size() - 1);
             const int kw = stored ? kStoredBwdK : window_k(tu);
             d.nwin = (int32_t)((b.ce - b.cs + kw - 1) / kw);
-            // float2 rows: one per checkpoint window, or two per 16-byte record of two core sites
-            const int64_t rows = stored ? 2 * (int64_t)((b.ce - b.cs + 1) / 2) : (int64_t)d.nwin;
+            // float2 rows: two per 16-byte record of two checkpoint windows, or of two
+            // core sites (stored states)
+            const int64_t rows = stored ? 2 * (int64_t)((b.ce - b.cs + 1) / 2) : 2 * (int64_t)((d.nwin + 1) / 2);
             require(bd.size() < (size_t)INT32_MAX, "too many block windows");
             // ck_off holds the item's state elements until phases place them
             for (int64_t t = 0; t < tiles; ++t) items.push_back({(int32_t)bd.size(), (int32_t)t, rows * tile_pairs});
@@ -1724,10 +1725,10 @@ int gsmc_checkpoint_bytes(gsmc_context* ctx, int64_t n_sites, int64_t core, int6
         require(ctx != nullptr && bytes != nullptr, "NULL argument");
         const int64_t tile_pairs = 64 * (int64_t)launch_np(ctx->tune, 0);
         const int64_t tiles = (n_pairs + tile_pairs - 1) / tile_pairs;
-        uint64_t windows = 0;  // state rows: one per checkpoint window, or per core site with K = 1
+        uint64_t windows = 0;  // state rows: one per checkpoint window (an even count per block), or per core site with K = 1
         for (const Block& b : make_blocks(n_sites, core > 0 ? core : std::max<int64_t>(n_sites, 1), core > 0 ? flank : 0))
             windows += ctx->tune.K == 1 ? (uint64_t)(2 * ((b.ce - b.cs + 1) / 2))
-                                        : (uint64_t)((b.ce - b.cs + ctx->tune.K - 1) / ctx->tune.K);
+                                        : (uint64_t)(2 * (((b.ce - b.cs + ctx->tune.K - 1) / ctx->tune.K + 1) / 2));
         *bytes = windows * (uint64_t)tiles * (uint64_t)tile_pairs * sizeof(float2);
     });
 }
