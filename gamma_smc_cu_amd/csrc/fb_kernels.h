@@ -960,7 +960,7 @@ __device__ __forceinline__ void backward_item(const FbArgs& a, int item, int lan
 
 
 // Forward states of the backward kernels: in LDS for K = 8 (recomputed
-// states), which takes the K = 8 mean kernel from 123 to 77 registers, so
+// states), which takes the K = 8 mean kernel from 123 to under 80 registers, so
 // six waves per SIMD instead of four hide the latency of each lookup's
 // gathers; in registers for K = 16 (its 8 KB of states per wave would cap
 // the waves per CU) and for stored states.
@@ -969,7 +969,7 @@ template <int K, int NP, bool ST> constexpr int kStateLds = kStatesInLds<K, ST> 
 
 // Waves per SIMD the backward kernel is built for (a floor for the register
 // allocator): five with one pair per lane and forward states in LDS (the
-// mean kernel needs 77 registers and gets six; the persistent and CI /
+// mean kernel needs under 80 registers and gets six; the persistent and CI /
 // posterior kernels are held to 96 registers, five waves), three with them
 // in registers (a few registers over would leave two), no floor otherwise.
 template <int K, int NP, bool ST>
