@@ -11,8 +11,8 @@ Prints one JSON line (rank 0). Besides the throughput it reports:
 * ``roofline``: algorithmic HBM bytes of the decode kernel per launch over
   the kernel's event-timed duration, against the 8 TB/s HBM peak;
 * ``cpu_baseline``: the CPU oracle (FP32 restatement of the same recursion,
-  OpenMP over pairs) timed on this host on a 1000-pair sample over all
-  sites, extrapolated per pair-site;
+  OpenMP over pairs) timed on this host on a bounded pair sample over all
+  sites (``--cpu-pairs``, 8,000 by default), reported per pair-site;
 * ``checks``: the sampled pairs decoded alone on the GPU are bit-identical
   to the oracle, and to the same pairs' columns of the full run.
 """
