@@ -4,12 +4,14 @@
 // job through the sites of one block window -- in order. Per block window
 // (core [cs, ce) padded to [ps, pe)):
 //   phase 1  forward filter over [ps, ce), storing the state at the start
-//            of every K-site window of the core to HBM (pair-contiguous);
+//            of every K-site window of the core to HBM (pair-contiguous,
+//            two windows per 16-byte record);
 //   phase 2  backward filter over the right flank [ce, pe) (no output);
 //   phase 3  windows of the core, last to first: recompute the K forward
-//            states of the window from its checkpoint into registers, then
-//            walk the window backward, combining forward and backward
-//            states into the posterior and storing it pair-contiguous.
+//            states of the window from its checkpoint (into LDS at K = 8,
+//            registers at K = 16), then walk the window backward,
+//            combining forward and backward states into the posterior and
+//            storing it pair-contiguous.
 // Het calls come from XOR of the two bitpacked haplotype rows; the
 // per-site transition is a bilinear lookup in the table layer selected by
 // the site's bp gap, which is the same for every lane.
@@ -125,7 +127,7 @@ struct FbArgs {
     const ItemDev* items;
     int64_t n_items;
     int* queue;                // persistent mode: next item counter (starts at 0)
-    float2* ck;                // checkpoints: per item, [nwin][64 NP]
+    float2* ck;                // checkpoint records: per item, [ceil(nwin / 2)][64 NP] float4 (as float2 pairs)
     float2* yb;                // backward state at each item's core end: per item, [64 NP]
     int* queue2;               // persistent mode: item counter of the second kernel
     int64_t slot_elems;        // fused stored-state kernel: float2 state elements of each wave's slot
@@ -331,8 +333,8 @@ template <int NP> struct Lanes {
     const uint64_t* rj[NP];
     uint32_t col[NP];   // byte offset of the pair in an output row
     bool live[NP];      // the pair exists (idle lanes shadow the job's last pair)
-    uint32_t ck_col;    // byte offset of the lane in a state row of 64 NP float2
-    float2* ck;         // state rows of the item: row w at ck + w * 64 NP
+    uint32_t ck_col;    // byte offset of the lane in a row of 64 NP float2 (a record row is two such)
+    float2* ck;         // state records of the item: record row t at ck + t * 128 NP
     int32_t grp0;       // 64-pair group of slot q = 0 (groups are tile * NP + q)
 };
 
