@@ -193,13 +193,15 @@ std::shared_ptr<DeviceTables> get_tables(int device, const char* path, int N, do
         std::vector<float> dev(dt->bytes / sizeof(float));
         auto slab = [&](const float* nodes, float* out) {
             for (uint32_t r = 0; r < kPairRows; ++r)
-                for (uint32_t q = 0; q < 50; ++q) {
-                    float* e = out + (size_t)(r * 50 + q) * 4;
-                    e[0] = nodes[2 * (r * 50 + q)];
-                    e[1] = nodes[2 * (r * 50 + q) + 1];
-                    e[2] = nodes[2 * ((r + 1) * 50 + q)];
-                    e[3] = nodes[2 * ((r + 1) * 50 + q) + 1];
-                }
+                for (uint32_t t = 0; t < kRowChunks; ++t)
+                    for (uint32_t i = 0; i < 4; ++i) {
+                        const uint32_t q = std::min<uint32_t>(3 * t + i, 49);  // entries past 49 are never read
+                        float* e = out + ((size_t)r * kRowBytes + t * 64 + i * 16) / sizeof(float);
+                        e[0] = nodes[2 * (r * 50 + q)];
+                        e[1] = nodes[2 * (r * 50 + q) + 1];
+                        e[2] = nodes[2 * ((r + 1) * 50 + q)];
+                        e[3] = nodes[2 * ((r + 1) * 50 + q) + 1];
+                    }
         };
         const size_t slab_f = kSlabBytes / sizeof(float), node_f = (size_t)kGridNodes * 2;
         for (size_t k = 0; k < (size_t)N * 2; ++k) {
