@@ -39,8 +39,8 @@ HBM_PEAK_GBS = 8000.0
 # commands is profiles/fb_decode_chromosome_counters.md. Reported as
 # roofline.traffic only when a run matches that workload and shape.
 MEASURED_TRAFFIC = {
-    ("chromosome", 8, 1, "static", False): 177.76e9,
-    ("chromosome", 8, 1, "persistent", False): 176.77e9,
+    ("chromosome", 8, 1, "static", False): 171.36e9,
+    ("chromosome", 8, 1, "persistent", False): 169.79e9,
 }
 
 WORKLOADS = {

@@ -35,14 +35,16 @@ constexpr int kGridNodes = 2550;
 // is the 16-byte {m, c at node (r, q); m, c at node (r + 1, q)}, so the four
 // corners of cell (r, q) are the 32 contiguous bytes of entries (r, q) and
 // (r, q + 1).
-// Each row of entries is laid out in 64-byte chunks of four entries, chunk
-// t holding entries 3t .. 3t + 3 (consecutive chunks overlap by one), so
-// the 32 bytes of any cell lie inside one aligned 64-byte chunk and never
-// straddle a cache line: 17 chunks, 1,088 bytes per row.
+// Each row of entries is laid out in 128-byte chunks of eight entries,
+// chunk t holding entries 7t .. 7t + 7 (consecutive chunks overlap by one),
+// so the 32 bytes of any cell lie inside one aligned 128-byte cache line:
+// 7 chunks, 896 bytes per row.
 constexpr uint32_t kPairRows = 50;
-constexpr uint32_t kRowChunks = 17;
-constexpr uint32_t kRowBytes = kRowChunks * 64;
-constexpr uint32_t kSlabBytes = kPairRows * kRowBytes;  // one slab: 54,400 bytes
+constexpr uint32_t kChunkEntries = 8;
+constexpr uint32_t kChunkStride = kChunkEntries - 1;
+constexpr uint32_t kRowChunks = 7;
+constexpr uint32_t kRowBytes = kRowChunks * kChunkEntries * 16;
+constexpr uint32_t kSlabBytes = kPairRows * kRowBytes;  // one slab: 44,800 bytes
 constexpr uint32_t kLayerBytes = 2 * kSlabBytes;        // hom slab then het slab
 constexpr uint32_t kDescLayerBits = 13;              // site descriptor: layer | chunks << 13
 constexpr uint32_t kDescLayerMask = (1u << kDescLayerBits) - 1;
@@ -192,9 +194,9 @@ __device__ __forceinline__ void lookup(const char* __restrict__ layer, uint32_t 
     const float lc = fminf(floorf(fc), 48.0f);
     const float wm = fm - lm;
     const float wc = fc - lc;
-    // entry (r, q) of the first corner pair at 16-byte unit r * 68 + q + q / 3
-    // (chunk q / 3, position q % 3 in it); all exact small integers in float
-    const float q3 = floorf(lc * 0.33333334f);
+    // entry (r, q) of the first corner pair at 16-byte unit r * 56 + q + q / 7
+    // (chunk q / 7, position q % 7 in it); all exact small integers in float
+    const float q3 = floorf(lc * 0.142857149f);  // lc div 7 for lc < 49 (the factor rounds above 1/7)
     const uint32_t off = ((uint32_t)fmaf(lm, (float)(kRowBytes / 16), lc + q3) << 4) + slab;
     const bool upper = (__lane_id() & 2u) != 0;
     // first gather: lanes 0, 1 their entry (r, q + 1); lanes 2, 3 entry (r, q) of lanes 0, 1

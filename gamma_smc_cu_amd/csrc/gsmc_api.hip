@@ -194,9 +194,9 @@ std::shared_ptr<DeviceTables> get_tables(int device, const char* path, int N, do
         auto slab = [&](const float* nodes, float* out) {
             for (uint32_t r = 0; r < kPairRows; ++r)
                 for (uint32_t t = 0; t < kRowChunks; ++t)
-                    for (uint32_t i = 0; i < 4; ++i) {
-                        const uint32_t q = std::min<uint32_t>(3 * t + i, 49);  // entries past 49 are never read
-                        float* e = out + ((size_t)r * kRowBytes + t * 64 + i * 16) / sizeof(float);
+                    for (uint32_t i = 0; i < kChunkEntries; ++i) {
+                        const uint32_t q = std::min<uint32_t>(kChunkStride * t + i, 49);  // entries past 49 are never read
+                        float* e = out + ((size_t)r * kRowBytes + (t * kChunkEntries + i) * 16) / sizeof(float);
                         e[0] = nodes[2 * (r * 50 + q)];
                         e[1] = nodes[2 * (r * 50 + q) + 1];
                         e[2] = nodes[2 * ((r + 1) * 50 + q)];
@@ -500,6 +500,18 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 // schedule).
 bool stored_states(const Tuning& tu, int flags) { return tu.K == 1 && !(flags & (GSMC_CI | GSMC_POSTERIOR)); }
 int window_k(const Tuning& tu) { return tu.K == 8 ? 8 : 16; }
+
+// The tuning one launch runs with: the context's, except that a per-site
+// summary over checkpoints of 8 sites runs the static schedule. Its
+// persistent backward kernel needs 96 registers (5 waves per SIMD) where the
+// static one needs 80 (6 waves), and measured 6% slower (134 against 126 ms
+// for the chromosome workload on one box).
+Tuning launch_tuning(const Tuning& t, int flags)
+{
+    Tuning u = t;
+    if ((flags & GSMC_SUMMARY) && !stored_states(t, flags) && t.K == 8 && t.np == 1) u.schedule = 0;
+    return u;
+}
 constexpr int kStoredBwdK = 4;
 constexpr int kStoredWaves = 6;
 
@@ -810,7 +822,7 @@ FbArgs make_args(gsmc_batch* b, const Plan& pl, int flags)
 // captured into a graph.
 void enqueue_kernels(gsmc_context* ctx, const Plan& pl, const FbArgs& a, int flags, hipStream_t stream)
 {
-    const Tuning& tu = ctx->tune;
+    const Tuning tu = launch_tuning(ctx->tune, flags);
     if (pl.fused_grid > 0) {
         hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(1), 0, stream, a.queue);
         launch(fused_kernel(flags), (unsigned)pl.fused_grid, a, stream);
@@ -848,7 +860,7 @@ void enqueue_kernels(gsmc_context* ctx, const Plan& pl, const FbArgs& a, int fla
 void launch_plan(gsmc_batch* b, const std::vector<JobSpec>& jobs, int flags, hipStream_t stream)
 {
     gsmc_context* ctx = b->ctx;
-    const Tuning tu = ctx->tune;
+    const Tuning tu = launch_tuning(ctx->tune, flags);
     const bool summary = (flags & GSMC_SUMMARY) != 0;
     Plan pl = make_plan(ctx, jobs, tu, launch_np(tu, flags), summary, stored_states(tu, flags), b->ck.bytes, flags);
     if (pl.n_items == 0) return;
@@ -1599,8 +1611,8 @@ int gsmc_batch_capture(gsmc_context* ctx, const gsmc_job* jobs, int64_t n_jobs, 
         g->ctx = ctx;
         g->buf = new_batch(ctx);
         const bool summary = (flags & GSMC_SUMMARY) != 0;
-        Plan pl = make_plan(ctx, specs, ctx->tune, launch_np(ctx->tune, flags), summary,
-                            stored_states(ctx->tune, flags), 0, flags);
+        const Tuning tu = launch_tuning(ctx->tune, flags);
+        Plan pl = make_plan(ctx, specs, tu, launch_np(tu, flags), summary, stored_states(tu, flags), 0, flags);
         require(pl.n_items > 0, "nothing to capture: every job is empty");
         ensure_buffers(g->buf.get(), pl, summary);
         HIP_OK(hipMemcpy(g->buf->meta.p, pl.bytes.data(), pl.bytes.size(), hipMemcpyHostToDevice));

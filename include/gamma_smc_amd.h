@@ -137,8 +137,9 @@ int gsmc_context_info(const gsmc_context* ctx, int64_t* n_haps, int64_t* n_sites
  * per resident wave. pairs_per_lane: 1 or 2
  * (default 1), i.e. 64 or 128 pairs per 64-lane workgroup, for the
  * mean-only kernel (CI / posterior launches use 1); schedule: 0 static grid
- * of work items, 1 persistent waves draining a device queue (default 0);
- * xcd_order: static grid ordered so each XCD walks a contiguous run of
+ * of work items, 1 persistent waves draining a device queue (default 0;
+ * summary launches at checkpoint_k 8 with one pair per lane always use the
+ * static grid, whose kernel runs more waves per SIMD); xcd_order: static grid ordered so each XCD walks a contiguous run of
  * items (default 0); min_waves: work items the host-output path puts in
  * one launch at least (default 8192). No reference counterpart (the
  * reference has fixed 256-thread launches). */
