@@ -45,6 +45,7 @@ EXPORTS = (
     "gsmc_memcpy_dtoh_async", "gsmc_host_alloc", "gsmc_host_free", "gsmc_event_query",
     "gsmc_batch_reserve", "gsmc_run_fb_summary", "gsmc_run_fwd", "gsmc_run_fwd_device", "gsmc_batch_capture",
     "gsmc_graph_launch", "gsmc_graph_destroy", "gsmc_context_set_state_budget", "gsmc_context_release_buffers",
+    "gsmc_run_fwd_states", "gsmc_get_device", "gsmc_batch_buffer_bytes",
 )
 
 
@@ -73,6 +74,8 @@ def _declare(L):
                               _vp, _vp, _vp],
         "gsmc_device_count": [ctypes.POINTER(ctypes.c_int)],
         "gsmc_set_device": [_i32],
+        "gsmc_get_device": [ctypes.POINTER(ctypes.c_int)],
+        "gsmc_run_fwd_states": [_vp, _vp, _vp, _i64, _vp, _vp],
         "gsmc_mem_info": [ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)],
         "gsmc_bitpack": [_vp, _i64, _i64, _vp],
         "gsmc_unpack": [_vp, _i64, _i64, _vp],
@@ -114,6 +117,7 @@ def _declare(L):
         "gsmc_host_free": [_vp],
         "gsmc_event_query": [_vp, ctypes.POINTER(ctypes.c_int)],
         "gsmc_batch_reserve": [_vp, ctypes.c_uint64, _i64, _i64],
+        "gsmc_batch_buffer_bytes": [_vp, ctypes.POINTER(ctypes.c_uint64)],
         "gsmc_run_fb_summary": [_vp, _vp, _vp, _i64, _i64, _i64, _vp, _vp, _vp],
         "gsmc_run_fwd": [_vp, _vp, _vp, _i64, _i32, _vp, _vp, _vp],
         "gsmc_run_fwd_device": [_vp, _vp, _vp, _i64, _i32, _vp, _vp, _vp, _vp],
@@ -177,8 +181,21 @@ def device_count():
     return n.value
 
 
-def set_device(device):
-    check(lib().gsmc_set_device(int(device)))
+def get_device_count():
+    """Number of HIP devices (``bindings.cpp:3635-3639``)."""
+    return device_count()
+
+
+def set_device(device_id):
+    """Make ``device_id`` the current device (``bindings.cpp:3630-3633``)."""
+    check(lib().gsmc_set_device(int(device_id)))
+
+
+def get_device():
+    """The current device (``bindings.cpp:3641-3645``)."""
+    d = ctypes.c_int()
+    check(lib().gsmc_get_device(ctypes.byref(d)))
+    return d.value
 
 
 def bitpack(G):
@@ -395,6 +412,19 @@ class FlowContext:
                                      _ptr(out.get("lower")), _ptr(out.get("upper"))))
         return out
 
+    def run_fwd_states(self, pairs):
+        """Forward states of the cached forward-backward walk over the full
+        sequence (``bindings.cpp:2851-2893``): ``mean_log10`` and
+        ``cv_log10``, site-major ``(n_sites, n_pairs)`` float32, the state
+        (log10 mean in scaled time, log10 CV) after each site."""
+        pi, pj = _split_pairs(pairs)
+        P = pi.shape[0]
+        out = {"mean_log10": np.zeros((self._S, P), np.float32), "cv_log10": np.zeros((self._S, P), np.float32)}
+        if P and self._S:
+            check(lib().gsmc_run_fwd_states(self._h, _ptr(pi), _ptr(pj), P, _ptr(out["mean_log10"]),
+                                            _ptr(out["cv_log10"])))
+        return out
+
     def run_fb_summary(self, pairs, core_block_sites=0, flank_sites=0):
         """Per-site mean, min and max of the posterior mean over ``pairs``,
         without per-pair outputs (``bindings.cpp:2695-2788``): returns
@@ -488,3 +518,28 @@ class FlowContext:
                                               _ptr(out.get("posterior_alpha")), _ptr(out.get("posterior_beta"))))
         out["blocks"] = blocks
         return out
+
+
+def gamma_smc_flow_cached_fb(G, positions, pairs, Ne=10000.0, mu=1.25e-8, rho=1e-8, flow_field_path=DEFAULT_PATH,
+                             mean_only=False, cache_steps=0):
+    """One-shot cached forward-backward over the full sequence
+    (``bindings.cpp:2303-2440``, binding ``:3618-3628``): a context for
+    ``(G, positions, Ne, mu, rho)`` -- the given rates, no heterozygosity
+    estimate -- decodes ``pairs`` and is released. Returns ``mean`` and,
+    unless ``mean_only``, ``lower`` / ``upper``, site-major float32."""
+    ctx = FlowContext(G, positions, Ne, mu, rho, flow_field_path, cache_steps)
+    try:
+        return ctx.run_fb(pairs, mean_only=mean_only)
+    finally:
+        ctx.close()
+
+
+def gamma_smc_flow_cached_fwd(G, positions, pairs, Ne=10000.0, mu=1.25e-8, rho=1e-8, flow_field_path=DEFAULT_PATH,
+                              mean_only=True, cache_steps=0):
+    """One-shot forward-only filter (binding ``bindings.cpp:3607-3616``):
+    ``FlowContext.run_fwd`` on a context released afterwards."""
+    ctx = FlowContext(G, positions, Ne, mu, rho, flow_field_path, cache_steps)
+    try:
+        return ctx.run_fwd(pairs, mean_only=mean_only)
+    finally:
+        ctx.close()

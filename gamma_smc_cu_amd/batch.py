@@ -56,6 +56,18 @@ class Workspace:
     def wait(self):
         _core.check(_core.lib().gsmc_batch_wait(self.handle))
 
+    def reserve(self, checkpoint_bytes, max_items, max_jobs):
+        """Size the buffers for the largest launch now; a reserved workspace
+        keeps its checkpoint buffer at this size (``gsmc_batch_reserve``)."""
+        _core.check(_core.lib().gsmc_batch_reserve(self.handle, int(checkpoint_bytes), int(max_items),
+                                                   int(max_jobs)))
+
+    def checkpoint_buffer_bytes(self):
+        """Bytes the checkpoint / stored-state buffer holds now."""
+        b = ctypes.c_uint64()
+        _core.check(_core.lib().gsmc_batch_buffer_bytes(self.handle, ctypes.byref(b)))
+        return int(b.value)
+
     def done(self):
         d = ctypes.c_int()
         _core.check(_core.lib().gsmc_batch_query(self.handle, ctypes.byref(d)))

@@ -1059,6 +1059,46 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES))) voi
     }
 }
 
+// Forward states of the full-sequence walk (the reference's run_fwd_states
+// debug path, bindings.cpp:2851-2893 over gamma_smc_flow.cu:567-609): one
+// lane per pair, the decode's forward segments from the prior, and the
+// state (m, c) after every site written site-major to out_m / out_c
+// [S][P]. The first site starts from prior[q0] and applies descriptor sd0
+// (its gap from position -1, full stretches folded into q0).
+__global__ __launch_bounds__(64) void fwd_states_kernel(FbArgs a, const int32_t* __restrict__ pi,
+                                                        const int32_t* __restrict__ pj, int64_t P, int64_t S,
+                                                        int32_t q0, uint32_t sd0, float* out_m, float* out_c)
+{
+    const int lane = (int)threadIdx.x;
+    const int64_t p = (int64_t)blockIdx.x * 64 + lane;
+    const bool live = p < P;
+    const int64_t pp = live ? p : P - 1;
+    const int64_t hi = min(max((int64_t)pi[pp], (int64_t)0), a.n_haps - 1);
+    const int64_t hj = min(max((int64_t)pj[pp], (int64_t)0), a.n_haps - 1);
+    Lanes<1> L;
+    L.ri[0] = a.packed + (size_t)hi * (size_t)a.W;
+    L.rj[0] = a.packed + (size_t)hj * (size_t)a.W;
+    const float2 x0 = load_const(a.prior + q0);
+    float m = x0.x, c = x0.y;
+    HetCache<1> h;
+    for (int64_t s = 0; s < S; s += 16) {
+        Window<16, 1> win;
+        load_window(a, L, s, win, h);
+        if (s == 0) win.d[0] = sd0;
+        const int64_t cnt = min((int64_t)16, S - s);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            if (k < cnt) {
+                fwd_segment(a.t, win.d[k], het_slab(win.bits[0], k), m, c);
+                if (live) {
+                    out_m[(size_t)(s + k) * (size_t)P + (size_t)p] = m;
+                    out_c[(size_t)(s + k) * (size_t)P + (size_t)p] = c;
+                }
+            }
+        }
+    }
+}
+
 // Summary epilogue: per site r of a job, the groups' partials in group
 // order -- the FP64 sum g = 0, 1, ... (starting from the running total of
 // earlier pair chunks unless `first`), min and max. The last chunk writes

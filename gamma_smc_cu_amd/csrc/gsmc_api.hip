@@ -87,11 +87,14 @@ struct DevBuf {
         p = nullptr;
         bytes = 0;
     }
-    void ensure(size_t n, int dev)
+    // quiesced: the caller has already waited for every queued use of the
+    // buffer (a workspace's buffers, after its completion event), so growing
+    // it needs no device-wide wait.
+    void ensure(size_t n, int dev, bool quiesced = false)
     {
         if (n <= bytes && p) return;
         // Growing a buffer that queued work may still read: let that work finish.
-        if (p) HIP_OK(hipDeviceSynchronize());
+        if (p && !quiesced) HIP_OK(hipDeviceSynchronize());
         release();
         device = dev;
         HIP_OK(hipMalloc(&p, std::max<size_t>(n, 256)));
@@ -285,6 +288,7 @@ struct Tuning {
 };
 
 struct gsmc_batch;
+struct HostSlot;
 
 struct gsmc_context {
     std::mutex mu;
@@ -301,15 +305,13 @@ struct gsmc_context {
     DevBuf desc_h2;            // forward-only path: per-site gap descriptors
     DevBuf fwd_buf;            // forward-only host path: pair chunk and outputs
     std::shared_ptr<DeviceTables> tabs;
-    // host-output path
+    // host-output path: output slots, each a decode stream with its own
+    // workspace and device rows, drained through pinned staging on `copy`
     DevBuf pairs;
-    DevBuf out_slot[2];
+    std::vector<std::unique_ptr<HostSlot>> slots;
     PinnedBuf stage[2];
     hipStream_t compute = nullptr, copy = nullptr;
-    hipEvent_t ev_kernel[2] = {nullptr, nullptr};  // slot written (compute -> copy)
-    hipEvent_t ev_free[2] = {nullptr, nullptr};    // slot drained (copy -> compute)
     hipEvent_t ev_stage[2] = {nullptr, nullptr};   // staging buffer filled (copy -> host)
-    std::unique_ptr<gsmc_batch> host_batch[2];
     std::unique_ptr<gsmc_batch> dev_batch;
     DevBuf sum_pairs, sum_site;  // per-site summary path: pair chunk, outputs and running values
 
@@ -327,6 +329,10 @@ struct gsmc_batch {
     hipStream_t last_stream = nullptr;
     hipEvent_t done = nullptr;
     bool pending = false;
+    // gsmc_batch_reserve sized the buffers: launches keep their state
+    // buffer within the reservation (more phases, or fewer resident
+    // waves) instead of growing it on the launch path.
+    bool reserved = false;
     ~gsmc_batch()
     {
         if (done) {
@@ -337,16 +343,32 @@ struct gsmc_batch {
     }
 };
 
+// One output slot of the host-output path: a decode stream, its launch
+// workspace and device rows, and the events that hand the rows to the copy
+// stream (kernel) and back (free).
+struct HostSlot {
+    std::unique_ptr<gsmc_batch> batch;
+    DevBuf out;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_kernel = nullptr;  // rows written (slot stream -> copy)
+    hipEvent_t ev_free = nullptr;    // rows drained (copy -> slot stream)
+    int device = 0;
+    ~HostSlot()
+    {
+        (void)hipSetDevice(device);
+        batch.reset();
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (ev_kernel) (void)hipEventDestroy(ev_kernel);
+        if (ev_free) (void)hipEventDestroy(ev_free);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
 gsmc_context::~gsmc_context()
 {
     (void)hipSetDevice(device);
     dev_batch.reset();
-    host_batch[0].reset();
-    host_batch[1].reset();
-    for (auto& e : ev_kernel)
-        if (e) (void)hipEventDestroy(e);
-    for (auto& e : ev_free)
-        if (e) (void)hipEventDestroy(e);
+    slots.clear();
     for (auto& e : ev_stage)
         if (e) (void)hipEventDestroy(e);
     if (compute) (void)hipStreamDestroy(compute);
@@ -361,11 +383,7 @@ void init_streams(gsmc_context* ctx)
     if (ctx->compute) return;
     HIP_OK(hipStreamCreateWithFlags(&ctx->compute, hipStreamNonBlocking));
     HIP_OK(hipStreamCreateWithFlags(&ctx->copy, hipStreamNonBlocking));
-    for (int i = 0; i < 2; ++i) {
-        HIP_OK(hipEventCreateWithFlags(&ctx->ev_kernel[i], hipEventDisableTiming));
-        HIP_OK(hipEventCreateWithFlags(&ctx->ev_free[i], hipEventDisableTiming));
-        HIP_OK(hipEventCreateWithFlags(&ctx->ev_stage[i], hipEventDisableTiming));
-    }
+    for (int i = 0; i < 2; ++i) HIP_OK(hipEventCreateWithFlags(&ctx->ev_stage[i], hipEventDisableTiming));
 }
 
 std::unique_ptr<gsmc_batch> new_batch(gsmc_context* ctx)
@@ -374,6 +392,21 @@ std::unique_ptr<gsmc_batch> new_batch(gsmc_context* ctx)
     b->ctx = ctx;
     HIP_OK(hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
     return b;
+}
+
+// At least n output slots of the host-output path.
+void ensure_slots(gsmc_context* ctx, int n)
+{
+    init_streams(ctx);
+    while ((int)ctx->slots.size() < n) {
+        auto sl = std::make_unique<HostSlot>();
+        sl->device = ctx->device;
+        HIP_OK(hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking));
+        HIP_OK(hipEventCreateWithFlags(&sl->ev_kernel, hipEventDisableTiming));
+        HIP_OK(hipEventCreateWithFlags(&sl->ev_free, hipEventDisableTiming));
+        sl->batch = new_batch(ctx);
+        ctx->slots.push_back(std::move(sl));
+    }
 }
 
 void finish_create(gsmc_context* ctx, const double* positions, double Ne, double mu, double rho, const char* path,
@@ -524,11 +557,13 @@ int64_t resident(const gsmc_context* ctx, const void* fn);
 
 // Bytes the state buffer of workspace b may take for a plan needing `want`:
 // all of it when that fits in the current allocation or in the allocation
-// plus free device memory (less a reserve), else that much.
-size_t state_budget(const gsmc_context* ctx, size_t have, size_t want)
+// plus free device memory (less a reserve), else that much. A reserved
+// workspace (fixed) never grows: the plan fits in what it holds.
+size_t state_budget(const gsmc_context* ctx, size_t have, size_t want, bool fixed)
 {
     const size_t cap = ctx->tune.max_state_bytes ? (size_t)ctx->tune.max_state_bytes : SIZE_MAX;
     if (want <= have && want <= cap) return want;
+    if (fixed) return std::min(std::min(want, have), cap);
     size_t free_b = 0, total_b = 0;
     HIP_OK(hipMemGetInfo(&free_b, &total_b));
     const size_t reserve = (size_t)2 << 30;
@@ -537,7 +572,7 @@ size_t state_budget(const gsmc_context* ctx, size_t have, size_t want)
 }
 
 Plan make_plan(const gsmc_context* ctx, const std::vector<JobSpec>& jobs, const Tuning& tu, int np,
-               bool summary, bool stored, size_t have_ck_bytes, int flags)
+               bool summary, bool stored, size_t have_ck_bytes, int flags, bool fixed_budget = false)
 {
     std::vector<JobDev> jd;
     std::vector<BlockDev> bd;
@@ -621,7 +656,8 @@ Plan make_plan(const gsmc_context* ctx, const std::vector<JobSpec>& jobs, const 
         slot = std::max<size_t>(slot, 1);
         const int64_t want = std::max<int64_t>(1, std::min<int64_t>(resident(ctx, fused_kernel(flags)),
                                                                     (int64_t)items.size()));
-        const size_t budget = state_budget(ctx, have_ck_bytes, (size_t)want * slot * sizeof(float2)) / sizeof(float2);
+        const size_t budget =
+            state_budget(ctx, have_ck_bytes, (size_t)want * slot * sizeof(float2), fixed_budget) / sizeof(float2);
         if (slot > budget)
             throw Error(GSMC_ENOMEM, "state budget below the forward states of one work item (" +
                                          std::to_string(slot * sizeof(float2)) + " bytes)");
@@ -646,7 +682,7 @@ Plan make_plan(const gsmc_context* ctx, const std::vector<JobSpec>& jobs, const 
     // Phases: when the states of all items exceed the budget, deal the
     // items round-robin (longest first) into the fewest phases that fit, so
     // every phase gets the same mix of long and short items.
-    const size_t budget = state_budget(ctx, have_ck_bytes, ck * sizeof(float2)) / sizeof(float2);
+    const size_t budget = state_budget(ctx, have_ck_bytes, ck * sizeof(float2), fixed_budget) / sizeof(float2);
     int64_t n_ph = 1;
     if (ck > budget) {
         size_t biggest = 0;
@@ -770,16 +806,19 @@ void launch(const void* fn, unsigned grid, const FbArgs& a, hipStream_t st)
     HIP_OK(hipLaunchKernel(fn, dim3(grid), dim3(64), args, 0, st));
 }
 
-// Buffers a planned launch needs on workspace b.
+// Buffers a planned launch needs on workspace b. Every caller has waited
+// for the workspace's previous launch (or the workspace is new), so no
+// queued work reads these buffers and growing them needs no device-wide
+// wait: a launch on one workspace never stalls on another's kernels.
 void ensure_buffers(gsmc_batch* b, const Plan& pl, bool summary)
 {
     gsmc_context* ctx = b->ctx;
-    b->meta.ensure(pl.bytes.size(), ctx->device);
-    b->ck.ensure(std::max<size_t>(pl.ck_elems, 1) * sizeof(float2), ctx->device);
-    b->queue.ensure(256, ctx->device);
-    b->yb.ensure((size_t)std::max<int64_t>(pl.n_items, 1) * 64 * 2 * sizeof(float2), ctx->device);
+    b->meta.ensure(pl.bytes.size(), ctx->device, true);
+    b->ck.ensure(std::max<size_t>(pl.ck_elems, 1) * sizeof(float2), ctx->device, true);
+    b->queue.ensure(256, ctx->device, true);
+    b->yb.ensure((size_t)std::max<int64_t>(pl.n_items, 1) * 64 * 2 * sizeof(float2), ctx->device, true);
     if (summary)
-        b->part.ensure(std::max<size_t>(pl.part_elems, 1) * (sizeof(double) + 2 * sizeof(float)), ctx->device);
+        b->part.ensure(std::max<size_t>(pl.part_elems, 1) * (sizeof(double) + 2 * sizeof(float)), ctx->device, true);
 }
 
 // Kernel arguments of a planned launch on workspace b's buffers.
@@ -862,7 +901,8 @@ void launch_plan(gsmc_batch* b, const std::vector<JobSpec>& jobs, int flags, hip
     gsmc_context* ctx = b->ctx;
     const Tuning tu = launch_tuning(ctx->tune, flags);
     const bool summary = (flags & GSMC_SUMMARY) != 0;
-    Plan pl = make_plan(ctx, jobs, tu, launch_np(tu, flags), summary, stored_states(tu, flags), b->ck.bytes, flags);
+    Plan pl = make_plan(ctx, jobs, tu, launch_np(tu, flags), summary, stored_states(tu, flags), b->ck.bytes, flags,
+                        b->reserved);
     if (pl.n_items == 0) return;
     const bool same = b->pending && stream == b->last_stream && pl.bytes == b->last;
     if (!same) {
@@ -930,35 +970,38 @@ void scatter_rows(float* dst, int64_t ld_dst, int64_t col0, const float* src, in
 }
 
 // Host-output driver shared by run_fb and run_fb_blockwise: pair chunks x
-// groups of consecutive blocks, each group decoded by one launch into a
-// device slot, then drained to host through pinned staging while the next
-// group runs.
+// groups of consecutive blocks, each group decoded by one launch into the
+// rows of an output slot, then drained to host through pinned staging while
+// the next groups run. n_slots slots (at least two) take the groups in
+// turn; each slot decodes on its own stream with its own workspace, so
+// launches of different slots may overlap on the device, and the copy
+// stream drains the slots in group order.
 void run_to_host(gsmc_context* ctx, const int32_t* pi, const int32_t* pj, int64_t P, const std::vector<Block>& blocks,
-                 int64_t pair_batch, int flags, float* const* host_out)
+                 int64_t pair_batch, int flags, float* const* host_out, int n_slots)
 {
     require(!(flags & (GSMC_Q8 | GSMC_Q4 | GSMC_SUMMARY)),
             "host-output runs write float rows; use gsmc_run_fb_device, a batch launch or gsmc_run_fb_summary");
-    init_streams(ctx);
-    for (auto& hb : ctx->host_batch)
-        if (!hb) hb = new_batch(ctx);
+    n_slots = std::max(2, n_slots);
+    ensure_slots(ctx, n_slots);
     const int nout = n_outputs(flags);
     const int64_t chunk = (pair_batch > 0) ? std::min<int64_t>(pair_batch, P) : P;
     // Blocks per launch: enough tile x block waves to fill the chip, bounded
-    // by a device budget for the two output slots.
+    // by a device budget for the output slots (half of free memory, shared).
     size_t free_b = 0, total_b = 0;
     HIP_OK(hipMemGetInfo(&free_b, &total_b));
+    size_t have_slots = 0;
+    for (int s = 0; s < n_slots; ++s) have_slots += ctx->slots[(size_t)s]->out.bytes;
     const int64_t tile_pairs = 64 * (int64_t)launch_np(ctx->tune, flags);
     const int64_t tiles = (chunk + tile_pairs - 1) / tile_pairs;
     int64_t per_group = std::max<int64_t>(1, (ctx->tune.min_waves + tiles - 1) / tiles);
     int64_t max_rows = 0;
     for (const Block& b : blocks) max_rows = std::max<int64_t>(max_rows, b.ce - b.cs);
-    const size_t slot_budget = (size_t)(free_b * 0.25);
+    const size_t slot_budget = (size_t)((double)(free_b + have_slots) * 0.5 / n_slots);
     while (per_group > 1 && (size_t)per_group * max_rows * chunk * 4 * nout > slot_budget) per_group /= 2;
 
     ctx->pairs.ensure((size_t)P * 8, ctx->device);
     int32_t* d_pi = ctx->pairs.as<int32_t>();
     int32_t* d_pj = d_pi + P;
-    HIP_OK(hipStreamSynchronize(ctx->compute));
     HIP_OK(hipMemcpy(d_pi, pi, (size_t)P * 4, hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(d_pj, pj, (size_t)P * 4, hipMemcpyHostToDevice));
 
@@ -986,14 +1029,14 @@ void run_to_host(gsmc_context* ctx, const int32_t* pi, const int32_t* pj, int64_
             const int64_t r0 = blocks[b0].cs, r1 = blocks[b0 + nb - 1].ce;
             tasks.push_back({p0, std::min(chunk, P - p0), b0, nb, r0, r1 - r0});
         }
-    for (int s = 0; s < 2; ++s) {
+    for (int s = 0; s < n_slots; ++s) {
         size_t slot_max = 0;
-        for (size_t i = s; i < tasks.size(); i += 2)
+        for (size_t i = (size_t)s; i < tasks.size(); i += (size_t)n_slots)
             slot_max = std::max(slot_max, (size_t)tasks[i].rows * tasks[i].np * 4 * nout);
-        if (slot_max) ctx->out_slot[s].ensure(slot_max, ctx->device);
+        if (slot_max) ctx->slots[(size_t)s]->out.ensure(slot_max, ctx->device);
     }
 
-    bool slot_used[2] = {false, false};
+    std::vector<char> slot_used((size_t)n_slots, 0);
     const size_t stage_bytes = (size_t)256 << 20;
     for (int s = 0; s < 2; ++s) ctx->stage[s].ensure(stage_bytes);
     int stage_next = 0;
@@ -1014,11 +1057,12 @@ void run_to_host(gsmc_context* ctx, const int32_t* pi, const int32_t* pj, int64_
 
     // copy one finished slot out through the staging ring, in row pieces
     auto copy_out = [&](int slot, const Task& t) {
+        HostSlot& sl = *ctx->slots[(size_t)slot];
         const size_t plane = (size_t)t.rows * t.np;
         const int64_t rows_per_piece = std::max<int64_t>(1, (int64_t)(stage_bytes / 4 / std::max<int64_t>(t.np, 1)));
-        HIP_OK(hipStreamWaitEvent(ctx->copy, ctx->ev_kernel[slot], 0));
+        HIP_OK(hipStreamWaitEvent(ctx->copy, sl.ev_kernel, 0));
         for (int o = 0; o < nout; ++o) {
-            const float* src = ctx->out_slot[slot].as<float>() + (size_t)o * plane;
+            const float* src = sl.out.as<float>() + (size_t)o * plane;
             for (int64_t r = 0; r < t.rows; r += rows_per_piece) {
                 const int64_t nr = std::min(rows_per_piece, t.rows - r);
                 const int s = stage_next;
@@ -1031,45 +1075,46 @@ void run_to_host(gsmc_context* ctx, const int32_t* pi, const int32_t* pj, int64_
                 stage_busy[s] = true;
             }
         }
-        HIP_OK(hipEventRecord(ctx->ev_free[slot], ctx->copy));
+        HIP_OK(hipEventRecord(sl.ev_free, ctx->copy));
     };
 
-    int prev_slot = -1;
-    const Task* prev_task = nullptr;
-    for (size_t i = 0; i < tasks.size(); ++i) {
-        const Task& t = tasks[i];
-        const int slot = (int)(i % 2);
-        const size_t plane = (size_t)t.rows * t.np;
-        if (slot_used[slot]) HIP_OK(hipStreamWaitEvent(ctx->compute, ctx->ev_free[slot], 0));
-        slot_used[slot] = true;
-        float* base = ctx->out_slot[slot].as<float>();
-        JobSpec js;
-        js.seq_lo = 0;
-        js.blocks.assign(blocks.begin() + t.b0, blocks.begin() + t.b0 + t.nb);
-        js.pi = d_pi + t.p0;
-        js.pj = d_pj + t.p0;
-        js.P = t.np;
-        js.row0 = t.row0;
-        int k = 0;
-        js.out.mean = base + plane * k++;
-        if (flags & GSMC_CI) {
-            js.out.lower = base + plane * k++;
-            js.out.upper = base + plane * k++;
+    // Groups are launched n_slots - 1 ahead of the copies: group i's copy is
+    // queued once group i + n_slots - 1 has been launched.
+    const size_t lag = (size_t)n_slots - 1;
+    for (size_t i = 0; i < tasks.size() + lag; ++i) {
+        if (i < tasks.size()) {
+            const Task& t = tasks[i];
+            const int slot = (int)(i % (size_t)n_slots);
+            HostSlot& sl = *ctx->slots[(size_t)slot];
+            const size_t plane = (size_t)t.rows * t.np;
+            if (slot_used[(size_t)slot]) HIP_OK(hipStreamWaitEvent(sl.stream, sl.ev_free, 0));
+            slot_used[(size_t)slot] = 1;
+            float* base = sl.out.as<float>();
+            JobSpec js;
+            js.seq_lo = 0;
+            js.blocks.assign(blocks.begin() + t.b0, blocks.begin() + t.b0 + t.nb);
+            js.pi = d_pi + t.p0;
+            js.pj = d_pj + t.p0;
+            js.P = t.np;
+            js.row0 = t.row0;
+            int k = 0;
+            js.out.mean = base + plane * k++;
+            if (flags & GSMC_CI) {
+                js.out.lower = base + plane * k++;
+                js.out.upper = base + plane * k++;
+            }
+            if (flags & GSMC_POSTERIOR) {
+                js.out.alpha = base + plane * k++;
+                js.out.beta = base + plane * k++;
+            }
+            launch_plan(sl.batch.get(), {js}, flags, sl.stream);
+            HIP_OK(hipEventRecord(sl.ev_kernel, sl.stream));
         }
-        if (flags & GSMC_POSTERIOR) {
-            js.out.alpha = base + plane * k++;
-            js.out.beta = base + plane * k++;
-        }
-        launch_plan(ctx->host_batch[slot].get(), {js}, flags, ctx->compute);
-        HIP_OK(hipEventRecord(ctx->ev_kernel[slot], ctx->compute));
-        if (prev_slot >= 0) copy_out(prev_slot, *prev_task);
-        prev_slot = slot;
-        prev_task = &t;
+        if (i >= lag) copy_out((int)((i - lag) % (size_t)n_slots), tasks[i - lag]);
     }
-    if (prev_slot >= 0) copy_out(prev_slot, *prev_task);
     drain_stage(stage_next);
     drain_stage(stage_next ^ 1);
-    HIP_OK(hipStreamSynchronize(ctx->compute));
+    for (int s = 0; s < n_slots; ++s) HIP_OK(hipStreamSynchronize(ctx->slots[(size_t)s]->stream));
     HIP_OK(hipStreamSynchronize(ctx->copy));
 }
 
@@ -1263,7 +1308,11 @@ int gsmc_context_release_buffers(gsmc_context* ctx)
         HIP_OK(hipSetDevice(ctx->device));
         if (ctx->compute) HIP_OK(hipStreamSynchronize(ctx->compute));
         if (ctx->copy) HIP_OK(hipStreamSynchronize(ctx->copy));
-        gsmc_batch* own[3] = {ctx->dev_batch.get(), ctx->host_batch[0].get(), ctx->host_batch[1].get()};
+        std::vector<gsmc_batch*> own{ctx->dev_batch.get()};
+        for (auto& sl : ctx->slots) {
+            HIP_OK(hipStreamSynchronize(sl->stream));
+            own.push_back(sl->batch.get());
+        }
         for (gsmc_batch* b : own) {
             if (!b) continue;
             if (b->pending) HIP_OK(hipEventSynchronize(b->done));
@@ -1274,8 +1323,7 @@ int gsmc_context_release_buffers(gsmc_context* ctx)
             b->yb.release();
             b->part.release();
         }
-        ctx->out_slot[0].release();
-        ctx->out_slot[1].release();
+        for (auto& sl : ctx->slots) sl->out.release();
         ctx->fwd_buf.release();
     });
 }
@@ -1324,7 +1372,7 @@ int gsmc_run_fb(gsmc_context* ctx, const int32_t* pi, const int32_t* pj, int64_t
         outs[k++] = mean;
         if (flags & GSMC_CI) outs[k++] = lower, outs[k++] = upper;
         if (flags & GSMC_POSTERIOR) outs[k++] = alpha, outs[k++] = beta;
-        run_to_host(ctx, pi, pj, P, blocks, -1, flags, outs);
+        run_to_host(ctx, pi, pj, P, blocks, -1, flags, outs, 2);
     });
 }
 
@@ -1355,7 +1403,7 @@ int gsmc_run_fb_blockwise(gsmc_context* ctx, const int32_t* pi, const int32_t* p
         outs[k++] = mean;
         if (flags & GSMC_CI) outs[k++] = lower, outs[k++] = upper;
         if (flags & GSMC_POSTERIOR) outs[k++] = alpha, outs[k++] = beta;
-        run_to_host(ctx, pi, pj, P, blocks, pair_batch, flags, outs);
+        run_to_host(ctx, pi, pj, P, blocks, pair_batch, flags, outs, max_streams);
     });
 }
 
@@ -1542,6 +1590,66 @@ int gsmc_run_fwd(gsmc_context* ctx, const int32_t* pi, const int32_t* pj, int64_
     });
 }
 
+int gsmc_run_fwd_states(gsmc_context* ctx, const int32_t* pi, const int32_t* pj, int64_t P, float* mean_log10,
+                        float* cv_log10)
+{
+    return guarded([&] {
+        require(ctx != nullptr, "context is NULL");
+        require(P >= 0, "negative n_pairs");
+        if (P == 0 || ctx->S == 0) return;
+        require(pi != nullptr && pj != nullptr && mean_log10 != nullptr && cv_log10 != nullptr, "NULL argument");
+        check_pairs(ctx, pi, pj, P);
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        HIP_OK(hipSetDevice(ctx->device));
+        init_streams(ctx);
+        const int64_t S = ctx->S;
+        // first site: its gap from position -1, full stretches from the prior table
+        const uint32_t full = descriptor(round_steps(ctx->pos[0] + 1.0), ctx->N);
+        const int64_t q = full >> kDescLayerBits;
+        const int32_t q0 = q < ctx->prior_n ? (int32_t)q : 0;
+        const uint32_t sd0 = q < ctx->prior_n ? (full & kDescLayerMask) : full;
+        // pairs per chunk: the two [S][chunk] planes within a quarter of free memory
+        size_t free_b = 0, total_b = 0;
+        HIP_OK(hipMemGetInfo(&free_b, &total_b));
+        int64_t chunk = (int64_t)(0.25 * (double)free_b / (8.0 * (double)S + 8.0)) / 64 * 64;
+        chunk = std::min<int64_t>(std::max<int64_t>(chunk, 64), (P + 63) / 64 * 64);
+        ctx->fwd_buf.ensure((size_t)chunk * 8 + 2 * (size_t)S * (size_t)chunk * 4, ctx->device);
+        int32_t* d_pi = ctx->fwd_buf.as<int32_t>();
+        int32_t* d_pj = d_pi + chunk;
+        float* d_m = reinterpret_cast<float*>(d_pj + chunk);
+        FbArgs a{};
+        a.packed = ctx->packed.as<uint64_t>();
+        a.n_haps = ctx->n;
+        a.W = ctx->W;
+        a.desc = ctx->desc.as<uint32_t>();
+        a.prior = ctx->prior.as<float2>();
+        a.t = ctx->tabs->view;
+        for (int64_t p0 = 0; p0 < P; p0 += chunk) {
+            const int64_t n = std::min(chunk, P - p0);
+            float* d_c = d_m + (size_t)S * n;
+            HIP_OK(hipStreamSynchronize(ctx->compute));
+            HIP_OK(hipMemcpyAsync(d_pi, pi + p0, (size_t)n * 4, hipMemcpyHostToDevice, ctx->compute));
+            HIP_OK(hipMemcpyAsync(d_pj, pj + p0, (size_t)n * 4, hipMemcpyHostToDevice, ctx->compute));
+            hipLaunchKernelGGL(fwd_states_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, ctx->compute, a,
+                               d_pi, d_pj, n, S, q0, sd0, d_m, d_c);
+            HIP_OK(hipGetLastError());
+            HIP_OK(hipMemcpy2DAsync(mean_log10 + p0, (size_t)P * 4, d_m, (size_t)n * 4, (size_t)n * 4, (size_t)S,
+                                    hipMemcpyDeviceToHost, ctx->compute));
+            HIP_OK(hipMemcpy2DAsync(cv_log10 + p0, (size_t)P * 4, d_c, (size_t)n * 4, (size_t)n * 4, (size_t)S,
+                                    hipMemcpyDeviceToHost, ctx->compute));
+        }
+        HIP_OK(hipStreamSynchronize(ctx->compute));
+    });
+}
+
+int gsmc_get_device(int* device)
+{
+    return guarded([&] {
+        require(device != nullptr, "NULL argument");
+        HIP_OK(hipGetDevice(device));
+    });
+}
+
 int gsmc_batch_create(gsmc_context* ctx, gsmc_batch** out)
 {
     return guarded([&] {
@@ -1565,11 +1673,13 @@ int gsmc_batch_reserve(gsmc_batch* b, uint64_t checkpoint_bytes, int64_t max_ite
         // A launch's metadata: jobs, then at most one block per item, then items.
         const size_t meta = align256((size_t)max_jobs * sizeof(JobDev)) +
                             align256((size_t)max_items * sizeof(BlockDev)) + (size_t)max_items * sizeof(ItemDev);
-        b->meta.ensure(meta, ctx->device);
+        b->pending = false;
+        b->meta.ensure(meta, ctx->device, true);
         b->stage.ensure(meta);
-        b->ck.ensure(std::max<uint64_t>(checkpoint_bytes, 1), ctx->device);
-        b->queue.ensure(256, ctx->device);
-        b->yb.ensure((size_t)std::max<int64_t>(max_items, 1) * 64 * 2 * sizeof(float2), ctx->device);
+        b->ck.ensure(std::max<uint64_t>(checkpoint_bytes, 1), ctx->device, true);
+        b->queue.ensure(256, ctx->device, true);
+        b->yb.ensure((size_t)std::max<int64_t>(max_items, 1) * 64 * 2 * sizeof(float2), ctx->device, true);
+        b->reserved = true;
     });
 }
 
@@ -1702,6 +1812,14 @@ int gsmc_batch_launch(gsmc_batch* b, const gsmc_job* jobs, int64_t n_jobs, int f
         std::lock_guard<std::mutex> lk(ctx->mu);
         HIP_OK(hipSetDevice(ctx->device));
         launch_plan(b, specs, flags, static_cast<hipStream_t>(stream));
+    });
+}
+
+int gsmc_batch_buffer_bytes(gsmc_batch* b, uint64_t* checkpoint_bytes)
+{
+    return guarded([&] {
+        require(b != nullptr && checkpoint_bytes != nullptr, "NULL argument");
+        *checkpoint_bytes = b->ck.bytes;
     });
 }
 

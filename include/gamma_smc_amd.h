@@ -82,9 +82,12 @@ const char* gsmc_last_error(void);
 int gsmc_build_tables(const char* flow_field_path, int n_steps, double Ne, double mu, double rho,
                       int table_precision, float* fwd, float* bwd, float* stretch);
 
-/* Number of HIP devices / select the current device (bindings.cpp:3630-3645 set_device). */
+/* Number of HIP devices, select / report the current device. Replace
+ * _core.get_device_count, set_device and get_device
+ * (src/bindings.cpp:3630-3645). */
 int gsmc_device_count(int* count);
 int gsmc_set_device(int device);
+int gsmc_get_device(int* device);
 
 /* Free and total bytes of the current device.
  * Replaces _core.cuda_mem_info (src/bindings.cpp:3646-3652). */
@@ -177,8 +180,12 @@ int gsmc_block_windows(int64_t n_sites, int64_t core_block_sites, int64_t flank_
 
 /* Blockwise forward-backward into host arrays [n_sites][n_pairs]: each
  * padded window decoded from the prior, core rows kept. pair_batch_size > 0
- * caps the pairs per launch (-1: all); max_streams >= 1 is the number of
- * output staging buffers in flight. Results do not depend on either.
+ * caps the pairs per launch (-1: all). max_streams >= 1 is the number of
+ * output slots, at least two: groups of blocks go to the slots in turn,
+ * each slot decoding on its own stream with its own workspace and device
+ * rows (launches of different slots may overlap on the device), while one
+ * copy stream drains finished slots to the host through pinned staging.
+ * Results do not depend on either.
  * Replaces FlowContext.run_fb_blockwise (src/bindings.cpp:2991-3345) and
  * gamma_smc_flow_cached_fb_block_gpu[_async] (gamma_smc_flow.cu:908-997). */
 int gsmc_run_fb_blockwise(gsmc_context* ctx, const int32_t* pair_i, const int32_t* pair_j, int64_t n_pairs,
@@ -208,6 +215,15 @@ int gsmc_run_fb_summary(gsmc_context* ctx, const int32_t* pair_i, const int32_t*
  * gamma_smc_flow_h2_fwd_gpu (src/kernels/gamma_smc_flow.cu:1139-1265). */
 int gsmc_run_fwd(gsmc_context* ctx, const int32_t* pair_i, const int32_t* pair_j, int64_t n_pairs, int flags,
                  float* mean, float* lower, float* upper);
+
+/* Forward states of the cached forward-backward walk (full sequence, FP32
+ * tables): after each site, the state's log10 mean (scaled time) into
+ * mean_log10 and its log10 CV into cv_log10, host arrays [n_sites][n_pairs].
+ * Replaces FlowContext.run_fwd_states (src/bindings.cpp:2851-2893, binding
+ * :3704) and gamma_smc_flow_cached_forward_states_gpu
+ * (src/kernels/gamma_smc_flow.cu:1542-1566). */
+int gsmc_run_fwd_states(gsmc_context* ctx, const int32_t* pair_i, const int32_t* pair_j, int64_t n_pairs,
+                        float* mean_log10, float* cv_log10);
 
 /* Device-resident form of gsmc_run_fwd: device pair ids and outputs,
  * queued on `stream` without synchronising. Same reference interface. */
@@ -252,8 +268,16 @@ void gsmc_graph_destroy(gsmc_graph* graph);
 
 /* Grow a batch's buffers now (checkpoint bytes, work items and jobs of the
  * largest launch the caller will make), so launches within these bounds
- * never allocate or synchronise the device. */
+ * never allocate or synchronise the device. A reserved batch never grows its
+ * checkpoint buffer: a launch needing more runs in phases (static schedule)
+ * or with fewer resident waves (persistent stored-state kernel). */
 int gsmc_batch_reserve(gsmc_batch* batch, uint64_t checkpoint_bytes, int64_t max_items, int64_t max_jobs);
+
+/* Bytes the batch's checkpoint / stored-state buffer holds now. After
+ * gsmc_batch_reserve, launches keep within the reservation (more phases or
+ * fewer resident waves), so this stays at the reserved size. No reference
+ * counterpart. */
+int gsmc_batch_buffer_bytes(gsmc_batch* batch, uint64_t* checkpoint_bytes);
 
 /* Device bytes of checkpoints one job of n_pairs pairs over n_sites sites
  * needs under the context's tuning (for admission control). */
