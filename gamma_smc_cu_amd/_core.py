@@ -45,7 +45,7 @@ EXPORTS = (
     "gsmc_memcpy_dtoh_async", "gsmc_host_alloc", "gsmc_host_free", "gsmc_event_query",
     "gsmc_batch_reserve", "gsmc_run_fb_summary", "gsmc_run_fwd", "gsmc_run_fwd_device", "gsmc_batch_capture",
     "gsmc_graph_launch", "gsmc_graph_destroy", "gsmc_context_set_state_budget", "gsmc_context_release_buffers",
-    "gsmc_run_fwd_states", "gsmc_get_device", "gsmc_batch_buffer_bytes",
+    "gsmc_run_fwd_states", "gsmc_get_device", "gsmc_batch_buffer_bytes", "gsmc_run_fb_device_noalloc",
 )
 
 
@@ -92,6 +92,7 @@ def _declare(L):
         "gsmc_block_windows": [_i64, _i64, _i64, _vp, _i64, ctypes.POINTER(_i64)],
         "gsmc_run_fb_blockwise": [_vp, _vp, _vp, _i64, _i64, _i64, _i64, _i32, _i32, _vp, _vp, _vp, _vp, _vp],
         "gsmc_run_fb_device": [_vp, _vp, _vp, _i64, _i64, _i64, _i32, _vp, _vp, _vp, _vp, _vp, _vp],
+        "gsmc_run_fb_device_noalloc": [_vp, _vp, _vp, _i64, _i64, _i64, _i32, _vp, _vp, _vp, _vp, _vp, _vp],
         "gsmc_malloc": [ctypes.POINTER(_vp), ctypes.c_uint64],
         "gsmc_free": [_vp],
         "gsmc_memcpy_htod": [_vp, _vp, ctypes.c_uint64],

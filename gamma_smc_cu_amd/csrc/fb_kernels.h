@@ -1059,6 +1059,43 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES))) voi
     }
 }
 
+// Launch metadata of one blockwise job, built on the device (for the entry
+// point that allocates nothing): the job, block windows b < nb of `core`
+// sites with `flank`-site flanks over [0, S), and items b * tiles + t with
+// a uniform state stride. A window starting at site 0 starts the sequence
+// (descriptor sd0 from prior[q0]); any other starts with descriptor
+// sd_restart (one bp). Same values as the host plan for this job.
+__global__ __launch_bounds__(256) void build_meta_kernel(JobDev job, JobDev* jobs, BlockDev* blocks, ItemDev* items,
+                                                         int64_t S, int64_t core, int64_t flank, int32_t nb,
+                                                         int32_t tiles, uint32_t sd0, int32_t q0, uint32_t sd_restart,
+                                                         int32_t K, int64_t ck_stride)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) jobs[0] = job;
+    if (i < nb) {
+        const int64_t cs = i * core, ce = min(S, cs + core);
+        BlockDev d;
+        d.cs = (int32_t)cs;
+        d.ce = (int32_t)ce;
+        d.ps = (int32_t)max((int64_t)0, cs - flank);
+        d.pe = (int32_t)min(S, ce + flank);
+        d.sd_fwd = d.ps == 0 ? sd0 : sd_restart;
+        d.sd_bwd = 0;
+        d.job = 0;
+        d.nwin = (int32_t)((ce - cs + K - 1) / K);
+        d.q0 = d.ps == 0 ? q0 : 0;
+        d.pad = 0;
+        blocks[i] = d;
+    }
+    if (i < (int64_t)nb * tiles) {
+        ItemDev it;
+        it.block = (int32_t)(i / tiles);
+        it.tile = (int32_t)(i % tiles);
+        it.ck_off = i * ck_stride;
+        items[i] = it;
+    }
+}
+
 // Forward states of the full-sequence walk (the reference's run_fwd_states
 // debug path, bindings.cpp:2851-2893 over gamma_smc_flow.cu:567-609): one
 // lane per pair, the decode's forward segments from the prior, and the

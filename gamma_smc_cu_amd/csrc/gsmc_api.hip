@@ -28,6 +28,13 @@
 #include "fb_kernels.h"
 #include "tables.h"
 
+// Device scratch of gsmc_run_fb_device_noalloc: launch metadata, flank
+// states and checkpoints of one call, for callers that cannot allocate.
+// Each device holds its own copy for the life of the process; host code
+// uses the per-device address from hipGetSymbolAddress, never this name.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) char scratch[kScratchBytes];
+
 namespace gsmc {
 namespace {
 
@@ -67,6 +74,24 @@ template <class F> int guarded(F&& f)
 void require(bool ok, const std::string& msg)
 {
     if (!ok) throw Error(GSMC_EINVAL, msg);
+}
+
+// Device address of `scratch` on the current device, looked up once per
+// device (each device holds its own copy at its own address).
+std::mutex g_scratch_mu;
+std::map<int, void*> g_scratch_at;
+
+void* scratch_address(int device)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_at.find(device);
+    if (it != g_scratch_at.end()) return it->second;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || ptr == nullptr)
+        throw Error(GSMC_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_at[device] = ptr;
+    return ptr;
 }
 
 // ------------------------------------------------------------- device buffers
@@ -314,6 +339,7 @@ struct gsmc_context {
     hipEvent_t ev_stage[2] = {nullptr, nullptr};   // staging buffer filled (copy -> host)
     std::unique_ptr<gsmc_batch> dev_batch;
     DevBuf sum_pairs, sum_site;  // per-site summary path: pair chunk, outputs and running values
+    void* scratch = nullptr;     // this device's `scratch` (gsmc_run_fb_device_noalloc)
 
     ~gsmc_context();
 };
@@ -1217,6 +1243,7 @@ static int create_common(const uint8_t* G, const uint64_t* packed, int64_t n, in
         require(S <= INT32_MAX - 1, "n_sites too large");
         ctx = new gsmc_context();
         HIP_OK(hipGetDevice(&ctx->device));
+        ctx->scratch = scratch_address(ctx->device);
         ctx->n = n;
         ctx->S = S;
         require(n <= INT32_MAX, "n_haps too large");
@@ -1432,6 +1459,97 @@ int gsmc_run_fb_device(gsmc_context* ctx, const int32_t* d_pi, const int32_t* d_
         js.P = P;
         js.out = Outputs{d_mean, d_lower, d_upper, d_alpha, d_beta};
         launch_plan(ctx->dev_batch.get(), {js}, flags, static_cast<hipStream_t>(stream));
+    });
+}
+
+int gsmc_run_fb_device_noalloc(gsmc_context* ctx, const int32_t* d_pi, const int32_t* d_pj, int64_t P, int64_t core,
+                               int64_t flank, int flags, float* d_mean, float* d_lower, float* d_upper, float* d_alpha,
+                               float* d_beta, void* stream)
+{
+    return guarded([&] {
+        require(ctx != nullptr, "context is NULL");
+        require(core > 0, "core_block_sites must be positive.");
+        require(flank >= 0, "flank_sites must be non-negative.");
+        require(P >= 0, "negative n_pairs");
+        if (P == 0 || ctx->S == 0) return;
+        require(d_pi != nullptr && d_pj != nullptr && d_mean != nullptr, "NULL argument");
+        require(!(flags & GSMC_CI) || (d_lower && d_upper), "lower/upper required with GSMC_CI");
+        require(!(flags & GSMC_POSTERIOR) || (d_alpha && d_beta), "alpha/beta required with GSMC_POSTERIOR");
+        require(!(flags & GSMC_SUMMARY), "use gsmc_run_fb_summary for per-site summaries");
+        const int64_t S = ctx->S;
+        const int64_t c = std::min(core, S);
+        const int64_t nb = (S + c - 1) / c;
+        require(!(flags & GSMC_Q4) || c % 2 == 0, "uint4 output needs an even core_block_sites");
+        require(!((flags & (GSMC_Q8 | GSMC_Q4)) && (flags & (GSMC_CI | GSMC_POSTERIOR))),
+                "quantized output cannot be combined with GSMC_CI or GSMC_POSTERIOR");
+        require(!((flags & GSMC_Q8) && (flags & GSMC_Q4)), "GSMC_Q8 and GSMC_Q4 are exclusive");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        HIP_OK(hipSetDevice(ctx->device));
+        require(ctx->scratch != nullptr, "no device scratch for this context");
+        // Checkpoints every 8 sites, one pair per lane, static grid: the
+        // same kernels (and so the same results) as gsmc_run_fb_device.
+        constexpr int K = 8;
+        const int64_t tiles = (P + 63) / 64;
+        const int64_t n_items = nb * tiles;
+        require(nb <= INT32_MAX && n_items <= INT32_MAX, "too many work items");
+        // scratch layout: job | blocks | items | flank states | checkpoints
+        const int64_t rows = 2 * (((c + K - 1) / K + 1) / 2);  // checkpoint float2 rows per item
+        const int64_t ck_stride = rows * 64;
+        const size_t off_blocks = align256(sizeof(JobDev));
+        const size_t off_items = off_blocks + align256((size_t)nb * sizeof(BlockDev));
+        const size_t off_yb = off_items + align256((size_t)n_items * sizeof(ItemDev));
+        const size_t off_ck = off_yb + align256((size_t)n_items * 64 * sizeof(float2));
+        const size_t need = off_ck + (size_t)n_items * (size_t)ck_stride * sizeof(float2);
+        if (need > kScratchBytes)
+            throw Error(GSMC_EINVAL, "gsmc_run_fb_device_noalloc: this call needs " + std::to_string(need) +
+                                         " bytes of device scratch, above its " + std::to_string(kScratchBytes) +
+                                         " byte maximum (fewer pairs or sites per call, or use gsmc_run_fb_device)");
+        // first site's gap from position -1 (full stretches from the prior
+        // table); block restarts step one bp from (position - 1)
+        const uint32_t full = descriptor(round_steps(ctx->pos[0] + 1.0), ctx->N);
+        const int64_t q = full >> kDescLayerBits;
+        const uint32_t sd0 = q < ctx->prior_n ? (full & kDescLayerMask) : full;
+        const int32_t q0 = q < ctx->prior_n ? (int32_t)q : 0;
+        uint32_t sd_restart = descriptor(1, ctx->N);
+        for (int64_t b = 0; b < nb; ++b) {
+            const int64_t ps = std::max<int64_t>(0, b * c - flank);
+            if (ps == 0) continue;
+            const double pp = ctx->pos[(size_t)ps];
+            require(descriptor(round_steps(pp - (pp - 1.0)), ctx->N) == sd_restart,
+                    "gsmc_run_fb_device_noalloc: positions too large for one-bp block restarts");
+        }
+        char* base = static_cast<char*>(ctx->scratch);
+        JobDev job{d_mean, d_lower, d_upper, d_alpha, d_beta, d_pi, d_pj, P, 0, S, -1};
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        const int64_t threads = std::max<int64_t>(n_items, nb);
+        hipLaunchKernelGGL(build_meta_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, job,
+                           reinterpret_cast<JobDev*>(base), reinterpret_cast<BlockDev*>(base + off_blocks),
+                           reinterpret_cast<ItemDev*>(base + off_items), S, c, flank, (int32_t)nb, (int32_t)tiles, sd0,
+                           q0, sd_restart, (int32_t)K, ck_stride);
+        HIP_OK(hipGetLastError());
+        FbArgs a{};
+        a.packed = ctx->packed.as<uint64_t>();
+        a.n_haps = ctx->n;
+        a.W = ctx->W;
+        a.desc = ctx->desc.as<uint32_t>();
+        a.prior = ctx->prior.as<float2>();
+        a.qbits = (flags & GSMC_Q8) ? 8 : (flags & GSMC_Q4) ? 4 : 0;
+        a.q_inv_range = 1.0f / logf(20.0f * ctx->Ne);
+        a.jobs = reinterpret_cast<const JobDev*>(base);
+        a.blocks = reinterpret_cast<const BlockDev*>(base + off_blocks);
+        a.items = reinterpret_cast<const ItemDev*>(base + off_items);
+        a.n_items = n_items;
+        a.yb = reinterpret_cast<float2*>(base + off_yb);
+        a.ck = reinterpret_cast<float2*>(base + off_ck);
+        a.t = ctx->tabs->view;
+        a.unscale = 2.0f * ctx->Ne;
+        Tuning tu;
+        tu.K = K;
+        tu.np = 1;
+        tu.schedule = 0;
+        const KernelPair kp = kernels_for(tu, flags);
+        launch(kp.fwd_items, (unsigned)n_items, a, st);
+        launch(kp.bwd_items, (unsigned)n_items, a, st);
     });
 }
 

@@ -241,6 +241,24 @@ int gsmc_run_fb_device(gsmc_context* ctx, const int32_t* d_pair_i, const int32_t
                        int64_t core_block_sites, int64_t flank_sites, int flags, float* d_mean, float* d_lower,
                        float* d_upper, float* d_alpha, float* d_beta, void* stream);
 
+/* gsmc_run_fb_device without any device allocation: the launch metadata,
+ * flank states and checkpoints of the call live in a fixed device array of
+ * 64 MiB (GSMC_NOALLOC_SCRATCH_BYTES) that each device holds for the life
+ * of the process; a small kernel writes the metadata, so the call makes no
+ * allocation, no host-to-device copy and no memset, only kernel launches
+ * on `stream`. Same arguments and the same results as gsmc_run_fb_device
+ * (checkpoints every 8 sites, static grid). A call needing more scratch
+ * than the maximum -- about n_blocks * ceil(n_pairs / 64) * (core / 8 + 2)
+ * * 512 bytes -- fails with GSMC_EINVAL before anything is queued. Every
+ * call on a device shares that device's array: calls whose work overlaps
+ * on one device (other streams, other threads, other contexts) race. No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops). */
+#define GSMC_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int gsmc_run_fb_device_noalloc(gsmc_context* ctx, const int32_t* d_pair_i, const int32_t* d_pair_j, int64_t n_pairs,
+                               int64_t core_block_sites, int64_t flank_sites, int flags, float* d_mean,
+                               float* d_lower, float* d_upper, float* d_alpha, float* d_beta, void* stream);
+
 /* Launch workspaces for batched ragged jobs and the serving loop: a batch
  * owns the metadata, checkpoint and queue buffers of one launch in flight.
  * gsmc_batch_launch queues every job of the list as one kernel launch on
