@@ -46,6 +46,7 @@ EXPORTS = (
     "gsmc_batch_reserve", "gsmc_run_fb_summary", "gsmc_run_fwd", "gsmc_run_fwd_device", "gsmc_batch_capture",
     "gsmc_graph_launch", "gsmc_graph_destroy", "gsmc_context_set_state_budget", "gsmc_context_release_buffers",
     "gsmc_run_fwd_states", "gsmc_get_device", "gsmc_batch_buffer_bytes", "gsmc_run_fb_device_noalloc",
+    "gsmc_lookup_probe",
 )
 
 
@@ -75,6 +76,7 @@ def _declare(L):
         "gsmc_device_count": [ctypes.POINTER(ctypes.c_int)],
         "gsmc_set_device": [_i32],
         "gsmc_get_device": [ctypes.POINTER(ctypes.c_int)],
+        "gsmc_lookup_probe": [_vp, _i64, _i32, _vp, _vp, ctypes.POINTER(ctypes.c_float)],
         "gsmc_run_fwd_states": [_vp, _vp, _vp, _i64, _vp, _vp],
         "gsmc_mem_info": [ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)],
         "gsmc_bitpack": [_vp, _i64, _i64, _vp],

@@ -11,9 +11,10 @@ fastest. Candidates sweep:
   walk, one lookup per site fewer for 16 bytes of HBM traffic per
   pair-site (two kernels per phase of items with the static schedule, one
   fused kernel with the persistent one);
-* ``pairs_per_lane`` -- 1 or 2 pairs per lane, i.e. 64 or 128 pairs per
-  64-lane workgroup (two pairs give every phase two independent lookup
-  chains per lane, at the cost of registers);
+* ``pairs_per_lane`` -- 1, 2 or 4 pairs per lane, i.e. 64, 128 or 256
+  pairs per 64-lane workgroup (more pairs give every phase more independent
+  lookup chains per lane, at the cost of registers and, at 4, of the LDS
+  that holds the recomputed forward states; 4 is built for checkpoint_k 8);
 * ``schedule`` -- a static grid of work items, ordered per XCD or not, or
   persistent waves draining a device queue.
 
@@ -31,7 +32,8 @@ CANDIDATES = [
     dict(checkpoint_k=k, pairs_per_lane=np_, schedule=sch, xcd_order=xcd)
     for k, np_, (sch, xcd) in itertools.product(
         (8, 16), (1, 2), (("static", False), ("static", True), ("persistent", False)))
-] + [dict(checkpoint_k=1, pairs_per_lane=1, schedule=sch, xcd_order=False) for sch in ("static", "persistent")]
+] + [dict(checkpoint_k=8, pairs_per_lane=4, schedule=sch, xcd_order=False) for sch in ("static", "persistent")] \
+  + [dict(checkpoint_k=1, pairs_per_lane=1, schedule=sch, xcd_order=False) for sch in ("static", "persistent")]
 
 
 def tune(ctx, run_once, candidates=None, repeats=1, log=None):

@@ -1059,6 +1059,44 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES))) voi
     }
 }
 
+// Gather-only probe of the texture path: every lane runs two independent
+// chains of the decode's table lookup on the context's forward tables, one
+// layer per step shared by the wave (drawn from a hash over the N layers)
+// and the het slab in about 15% of lanes, with nothing else in the loop.
+// Its lookups per second bound what the decode's lookups can reach on this
+// device. Writes one float per lane so the chains are kept.
+__device__ __forceinline__ uint32_t probe_hash(uint32_t x)
+{
+    x ^= x >> 16;
+    x *= 0x7feb352dU;
+    x ^= x >> 15;
+    x *= 0x846ca68bU;
+    x ^= x >> 16;
+    return x;
+}
+
+__global__ __launch_bounds__(64) void lookup_probe_kernel(DevTables t, int32_t n_layers, int32_t steps, float* out)
+{
+    const uint32_t gid = blockIdx.x * 64u + threadIdx.x;
+    float m[2], c[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const uint32_t h = probe_hash(gid * 7919u + (uint32_t)k * 104729u + 17u);
+        m[k] = -1.5f + 2.5f * (float)(h & 0xFFFFu) * (1.0f / 65535.0f);
+        c[k] = -1.2f + 0.9f * (float)(h >> 16) * (1.0f / 65535.0f);
+    }
+    for (int32_t r = 0; r < steps; ++r) {
+        const uint32_t lay = __builtin_amdgcn_readfirstlane(probe_hash((uint32_t)r * 31u + blockIdx.x) % (uint32_t)n_layers);
+        const char* layer = t.fwd + (size_t)lay * kLayerBytes;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const uint32_t hs = probe_hash(gid * 13u + (uint32_t)r * 7u + (uint32_t)k);
+            lookup(layer, (hs & 1023u) < 159u ? kSlabBytes : 0u, m[k], c[k]);
+        }
+    }
+    out[gid] = m[0] + c[0] + m[1] + c[1];
+}
+
 // Launch metadata of one blockwise job, built on the device (for the entry
 // point that allocates nothing): the job, block windows b < nb of `core`
 // sites with `flank`-site flanks over [0, S), and items b * tiles + t with

@@ -773,7 +773,9 @@ Plan make_plan(const gsmc_context* ctx, const std::vector<JobSpec>& jobs, const 
 // are built for 1 and 2, the CI / posterior kernels for 1.
 int launch_np(const Tuning& tu, int flags)
 {
-    return ((flags & (GSMC_CI | GSMC_POSTERIOR)) || tu.K == 1) ? 1 : tu.np;
+    if ((flags & (GSMC_CI | GSMC_POSTERIOR)) || tu.K == 1) return 1;
+    // four pairs per lane: mean / code kernels only; summaries take two
+    return (flags & GSMC_SUMMARY) ? std::min(tu.np, 2) : tu.np;
 }
 
 // Kernel pair of one launch shape, and each kernel's resident waves.
@@ -799,6 +801,8 @@ template <int K> KernelPair kernels_k(const Tuning& tu, int flags)
     if (ci && post) return kernels_of<K, 1, true, true>();
     if (ci) return kernels_of<K, 1, true, false>();
     if (post) return kernels_of<K, 1, false, true>();
+    if constexpr (K == 8)
+        if (tu.np == 4) return kernels_of<K, 4, false, false>();
     if (tu.np == 2) return kernels_of<K, 2, false, false>();
     return kernels_of<K, 1, false, false>();
 }
@@ -1315,7 +1319,8 @@ int gsmc_context_set_tuning(gsmc_context* ctx, int k, int pairs_per_lane, int sc
     return guarded([&] {
         require(ctx != nullptr, "context is NULL");
         require(k == 1 || k == 8 || k == 16, "checkpoint_k must be 1 (stored forward states), 8 or 16");
-        require(pairs_per_lane == 1 || pairs_per_lane == 2, "pairs_per_lane must be 1 or 2");
+        require(pairs_per_lane == 1 || pairs_per_lane == 2 || pairs_per_lane == 4, "pairs_per_lane must be 1, 2 or 4");
+        require(pairs_per_lane != 4 || k == 8, "pairs_per_lane 4 needs checkpoint_k 8");
         require(schedule == 0 || schedule == 1, "schedule must be 0 (static) or 1 (persistent)");
         require(min_waves >= 1, "min_waves_per_launch must be positive");
         std::lock_guard<std::mutex> lk(ctx->mu);
@@ -1757,6 +1762,34 @@ int gsmc_run_fwd_states(gsmc_context* ctx, const int32_t* pi, const int32_t* pj,
                                     hipMemcpyDeviceToHost, ctx->compute));
         }
         HIP_OK(hipStreamSynchronize(ctx->compute));
+    });
+}
+
+int gsmc_lookup_probe(gsmc_context* ctx, int64_t waves, int32_t steps, void* out_scratch, void* stream, float* ms)
+{
+    return guarded([&] {
+        require(ctx != nullptr && out_scratch != nullptr && ms != nullptr, "NULL argument");
+        require(waves > 0 && waves <= INT32_MAX / 64 && steps > 0, "bad probe shape");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        HIP_OK(hipSetDevice(ctx->device));
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        hipEvent_t e0, e1;
+        HIP_OK(hipEventCreate(&e0));
+        HIP_OK(hipEventCreate(&e1));
+        float* out = static_cast<float*>(out_scratch);
+        // one untimed launch (page-in), then the timed one
+        hipLaunchKernelGGL(lookup_probe_kernel, dim3((unsigned)waves), dim3(64), 0, st, ctx->tabs->view, ctx->N,
+                           std::min<int32_t>(steps, 64), out);
+        HIP_OK(hipEventRecord(e0, st));
+        hipLaunchKernelGGL(lookup_probe_kernel, dim3((unsigned)waves), dim3(64), 0, st, ctx->tabs->view, ctx->N, steps,
+                           out);
+        HIP_OK(hipEventRecord(e1, st));
+        const hipError_t le = hipGetLastError();
+        HIP_OK(hipEventSynchronize(e1));
+        HIP_OK(hipEventElapsedTime(ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        HIP_OK(le);
     });
 }
 

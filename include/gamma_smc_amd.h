@@ -137,9 +137,10 @@ int gsmc_context_info(const gsmc_context* ctx, int64_t* n_haps, int64_t* n_sites
  * posterior launches use 16). With the static schedule that takes 8 B per
  * pair and core site of device memory, in launches split into phases that
  * fit; with the persistent schedule one fused kernel reuses a state slot
- * per resident wave. pairs_per_lane: 1 or 2
- * (default 1), i.e. 64 or 128 pairs per 64-lane workgroup, for the
- * mean-only kernel (CI / posterior launches use 1); schedule: 0 static grid
+ * per resident wave. pairs_per_lane: 1, 2 or 4 (4 with checkpoint_k 8)
+ * (default 1), i.e. 64, 128 or 256 pairs per 64-lane workgroup, for the
+ * mean-only kernel (CI / posterior launches use 1, summaries at most 2);
+ * schedule: 0 static grid
  * of work items, 1 persistent waves draining a device queue (default 0;
  * summary launches at checkpoint_k 8 with one pair per lane always use the
  * static grid, whose kernel runs more waves per SIMD); xcd_order: static grid ordered so each XCD walks a contiguous run of
@@ -325,6 +326,15 @@ int gsmc_event_destroy(void* event);
 int gsmc_event_record(void* event, void* stream);
 int gsmc_event_query(void* event, int* done); /* done = 1 once the event completed, without blocking */
 int gsmc_event_elapsed_ms(void* start, void* stop, float* ms);
+/* Gather-only probe of the texture path: `waves` 64-lane waves each run two
+ * independent chains of `steps` table lookups on the context's forward
+ * tables (one layer per step shared by the wave, het slab in ~15% of lanes,
+ * nothing else in the loop); *ms is the timed launch's duration, so
+ * 2 * 64 * waves * steps / ms is the lookup rate the device sustains. out
+ * is device memory for 64 * waves floats. No reference counterpart
+ * (measurement). */
+int gsmc_lookup_probe(gsmc_context* ctx, int64_t waves, int32_t steps, void* out, void* stream, float* ms);
+
 /* Streaming device-to-device copy kernel (bytes a multiple of 16), queued
  * on `stream`; a kernel of known traffic for counter calibration. */
 int gsmc_device_copy(void* dst, const void* src, uint64_t bytes, void* stream);
