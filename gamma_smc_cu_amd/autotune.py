@@ -33,7 +33,8 @@ CANDIDATES = [
     for k, np_, (sch, xcd) in itertools.product(
         (8, 16), (1, 2), (("static", False), ("static", True), ("persistent", False)))
 ] + [dict(checkpoint_k=8, pairs_per_lane=4, schedule=sch, xcd_order=False) for sch in ("static", "persistent")] \
-  + [dict(checkpoint_k=1, pairs_per_lane=1, schedule=sch, xcd_order=False) for sch in ("static", "persistent")]
+  + [dict(checkpoint_k=1, pairs_per_lane=np_, schedule="static", xcd_order=False) for np_ in (1, 2)] \
+  + [dict(checkpoint_k=1, pairs_per_lane=1, schedule="persistent", xcd_order=False)]
 
 
 def tune(ctx, run_once, candidates=None, repeats=1, log=None):

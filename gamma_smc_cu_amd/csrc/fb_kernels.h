@@ -983,9 +983,11 @@ template <int K, int NP, bool ST> constexpr int kStateLds = kStatesInLds<K, ST> 
 // allocator): five with one pair per lane and forward states in LDS (the
 // mean kernel needs under 80 registers and gets six; the persistent and CI /
 // posterior kernels are held to 96 registers, five waves), three with them
-// in registers (a few registers over would leave two), no floor otherwise.
+// in registers (a few registers over would leave two), four for stored
+// states with two pairs per lane (two lookup chains per wave; 134 registers
+// unbounded, three waves), no floor otherwise.
 template <int K, int NP, bool ST>
-constexpr int kBwdWaves = (NP == 1 && kStatesInLds<K, ST>) ? 5 : (NP == 1 && !ST) ? 3 : 1;
+constexpr int kBwdWaves = (NP == 1 && kStatesInLds<K, ST>) ? 5 : (NP == 1 && !ST) ? 3 : (NP == 2 && ST) ? 4 : 1;
 
 // LDS floats of the backward kernel: the summary's staging rows (one row
 // of 65 per site of the window), or the [K][64] tile of float-mean row

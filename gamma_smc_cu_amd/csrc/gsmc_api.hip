@@ -546,7 +546,11 @@ struct Plan {
     std::vector<Phase> phases;
     int64_t fused_grid = 0;   // stored states: resident waves of the fused kernel
     int64_t slot_elems = 0;   // ... and the float2 state elements of each wave's slot
+    int np = 1;               // pairs per lane of the plan's tiles (kernels index 64 np lanes per item)
 };
+
+// Most pairs per lane any kernel is built for (sizes of reserved buffers).
+constexpr int kMaxPairsPerLane = 4;
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
@@ -662,6 +666,7 @@ Plan make_plan(const gsmc_context* ctx, const std::vector<JobSpec>& jobs, const 
         }
     }
     Plan pl;
+    pl.np = np;
     pl.n_waves = (int64_t)items.size();
     // Longest chains first: an item's duration is its walk length (forward
     // over [ps, ce), backward over the right flank, two lookups per core
@@ -773,7 +778,10 @@ Plan make_plan(const gsmc_context* ctx, const std::vector<JobSpec>& jobs, const 
 // are built for 1 and 2, the CI / posterior kernels for 1.
 int launch_np(const Tuning& tu, int flags)
 {
-    if ((flags & (GSMC_CI | GSMC_POSTERIOR)) || tu.K == 1) return 1;
+    if (flags & (GSMC_CI | GSMC_POSTERIOR)) return 1;
+    // stored forward states: one or two pairs per lane with the static
+    // schedule, one in the fused persistent kernel
+    if (tu.K == 1) return tu.schedule == 1 ? 1 : std::min(tu.np, 2);
     // four pairs per lane: mean / code kernels only; summaries take two
     return (flags & GSMC_SUMMARY) ? std::min(tu.np, 2) : tu.np;
 }
@@ -794,31 +802,36 @@ template <int K, int NP, bool C, bool Q, bool SUM = false, bool ST = false> Kern
             reinterpret_cast<const void*>(&fb_bwd_queue<K, NP, C, Q, SUM, ST>)};
 }
 
+// The kernels for the pairs per lane launch_np plans with (the tile size of
+// the plan and the kernel's NP must agree).
 template <int K> KernelPair kernels_k(const Tuning& tu, int flags)
 {
     const bool ci = flags & GSMC_CI, post = flags & GSMC_POSTERIOR;
-    if (flags & GSMC_SUMMARY) return tu.np == 2 ? kernels_of<K, 2, false, false, true>() : kernels_of<K, 1, false, false, true>();
+    const int np = launch_np(tu, flags);
+    if (flags & GSMC_SUMMARY) return np == 2 ? kernels_of<K, 2, false, false, true>() : kernels_of<K, 1, false, false, true>();
     if (ci && post) return kernels_of<K, 1, true, true>();
     if (ci) return kernels_of<K, 1, true, false>();
     if (post) return kernels_of<K, 1, false, true>();
     if constexpr (K == 8)
-        if (tu.np == 4) return kernels_of<K, 4, false, false>();
-    if (tu.np == 2) return kernels_of<K, 2, false, false>();
+        if (np == 4) return kernels_of<K, 4, false, false>();
+    if (np == 2) return kernels_of<K, 2, false, false>();
     return kernels_of<K, 1, false, false>();
+}
+
+template <int NP, bool SUM> KernelPair stored_kernels()
+{
+    return {reinterpret_cast<const void*>(&fb_fwd_items<16, NP, true>),
+            reinterpret_cast<const void*>(&fb_bwd_items<kStoredBwdK, NP, false, false, SUM, true>),
+            reinterpret_cast<const void*>(&fb_fwd_queue<16, NP, true>),
+            reinterpret_cast<const void*>(&fb_bwd_queue<kStoredBwdK, NP, false, false, SUM, true>)};
 }
 
 KernelPair kernels_for(const Tuning& tu, int flags)
 {
     if (stored_states(tu, flags)) {
-        if (flags & GSMC_SUMMARY)
-            return {reinterpret_cast<const void*>(&fb_fwd_items<16, 1, true>),
-                    reinterpret_cast<const void*>(&fb_bwd_items<kStoredBwdK, 1, false, false, true, true>),
-                    reinterpret_cast<const void*>(&fb_fwd_queue<16, 1, true>),
-                    reinterpret_cast<const void*>(&fb_bwd_queue<kStoredBwdK, 1, false, false, true, true>)};
-        return {reinterpret_cast<const void*>(&fb_fwd_items<16, 1, true>),
-                reinterpret_cast<const void*>(&fb_bwd_items<kStoredBwdK, 1, false, false, false, true>),
-                reinterpret_cast<const void*>(&fb_fwd_queue<16, 1, true>),
-                reinterpret_cast<const void*>(&fb_bwd_queue<kStoredBwdK, 1, false, false, false, true>)};
+        const bool sum = (flags & GSMC_SUMMARY) != 0;
+        if (launch_np(tu, flags) == 2) return sum ? stored_kernels<2, true>() : stored_kernels<2, false>();
+        return sum ? stored_kernels<1, true>() : stored_kernels<1, false>();
     }
     return window_k(tu) == 8 ? kernels_k<8>(tu, flags) : kernels_k<16>(tu, flags);
 }
@@ -836,6 +849,9 @@ void launch(const void* fn, unsigned grid, const FbArgs& a, hipStream_t st)
     HIP_OK(hipLaunchKernel(fn, dim3(grid), dim3(64), args, 0, st));
 }
 
+// Flank states of n_items items of 64 np pairs each (one float2 per pair).
+size_t yb_bytes(int64_t n_items, int np) { return (size_t)std::max<int64_t>(n_items, 1) * 64 * (size_t)np * sizeof(float2); }
+
 // Buffers a planned launch needs on workspace b. Every caller has waited
 // for the workspace's previous launch (or the workspace is new), so no
 // queued work reads these buffers and growing them needs no device-wide
@@ -846,7 +862,7 @@ void ensure_buffers(gsmc_batch* b, const Plan& pl, bool summary)
     b->meta.ensure(pl.bytes.size(), ctx->device, true);
     b->ck.ensure(std::max<size_t>(pl.ck_elems, 1) * sizeof(float2), ctx->device, true);
     b->queue.ensure(256, ctx->device, true);
-    b->yb.ensure((size_t)std::max<int64_t>(pl.n_items, 1) * 64 * 2 * sizeof(float2), ctx->device, true);
+    b->yb.ensure(yb_bytes(pl.n_items, pl.np), ctx->device, true);
     if (summary)
         b->part.ensure(std::max<size_t>(pl.part_elems, 1) * (sizeof(double) + 2 * sizeof(float)), ctx->device, true);
 }
@@ -855,6 +871,9 @@ void ensure_buffers(gsmc_batch* b, const Plan& pl, bool summary)
 FbArgs make_args(gsmc_batch* b, const Plan& pl, int flags)
 {
     gsmc_context* ctx = b->ctx;
+    // the kernels index these buffers by the plan: refuse a launch they cannot hold
+    require(b->yb.bytes >= yb_bytes(pl.n_items, pl.np) && b->ck.bytes >= pl.ck_elems * sizeof(float2),
+            "internal: launch buffers smaller than the plan");
     FbArgs a;
     a.packed = ctx->packed.as<uint64_t>();
     a.n_haps = ctx->n;
@@ -1829,7 +1848,7 @@ int gsmc_batch_reserve(gsmc_batch* b, uint64_t checkpoint_bytes, int64_t max_ite
         b->stage.ensure(meta);
         b->ck.ensure(std::max<uint64_t>(checkpoint_bytes, 1), ctx->device, true);
         b->queue.ensure(256, ctx->device, true);
-        b->yb.ensure((size_t)std::max<int64_t>(max_items, 1) * 64 * 2 * sizeof(float2), ctx->device, true);
+        b->yb.ensure(yb_bytes(max_items, kMaxPairsPerLane), ctx->device, true);
         b->reserved = true;
     });
 }

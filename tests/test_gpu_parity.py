@@ -230,9 +230,10 @@ def test_batch_matches_alone_and_oracle(core, cohort_small, flow_field_path, tu)
             assert _bits_equal(a[k], ref[k]), k
 
 
-@pytest.mark.parametrize("schedule,budget", [("static", 0), ("persistent", 0), ("static", 400_000),
-                                             ("persistent", 800_000)])
-def test_stored_forward_states_bit_exact(core, cohort_small, flow_field_path, schedule, budget):
+@pytest.mark.parametrize("schedule,budget,np_", [("static", 0, 1), ("persistent", 0, 1), ("static", 400_000, 1),
+                                                 ("persistent", 800_000, 1), ("static", 0, 2),
+                                                 ("static", 800_000, 2)])
+def test_stored_forward_states_bit_exact(core, cohort_small, flow_field_path, schedule, budget, np_):
     """checkpoint_k=1: each wave's forward walk stores the state after
     every core site in the wave's slot and its backward walk reads it back
     instead of recomputing it. Means, log codes and ragged batches equal the
@@ -245,9 +246,9 @@ def test_stored_forward_states_bit_exact(core, cohort_small, flow_field_path, sc
     pairs = O.all_pairs(G.shape[0]) + [(3, 3)]
     T = _tables(flow_field_path, 1.25e-8, 1e-8)
     ctx = core.FlowContext(G, pos, 1e4, 1.25e-8, 1e-8, flow_field_path, 0)
-    ctx.set_tuning(checkpoint_k=1, pairs_per_lane=1, schedule=schedule)
+    ctx.set_tuning(checkpoint_k=1, pairs_per_lane=np_, schedule=schedule)
     ctx.set_state_budget(budget)
-    _case(f"stored states {schedule} budget={budget}: full sequence")
+    _case(f"stored states {schedule} budget={budget} pairs/lane={np_}: full sequence")
     got = ctx.run_fb(pairs, mean_only=True)
     assert _bits_equal(got["mean"], O.fb_window_f32(G, pos, pairs, T, 1e4)["mean"])
     for csz, fl in ((200, 64), (37, 5), (1, 3)):
@@ -274,6 +275,43 @@ def test_stored_forward_states_bit_exact(core, cohort_small, flow_field_path, sc
         ref = O.fb_blockwise_f32(np.ascontiguousarray(G[:, j.site_lo:j.site_hi]),
                                  np.ascontiguousarray(pos[j.site_lo:j.site_hi]), pairs_j, T, 1e4, core_sites, flank)
         assert _bits_equal(a["mean"], ref["mean"])
+
+
+@pytest.mark.parametrize("schedule", ["static", "persistent"])
+def test_four_pairs_per_lane_bit_exact(core, cohort_small, flow_field_path, schedule):
+    """checkpoint_k 8 with four pairs per lane (256 pairs per wave): means,
+    log codes, per-site summaries (which run two pairs per lane) and a
+    ragged batch equal the oracle's, also with a partial last tile."""
+    from gamma_smc_cu_amd import batch
+
+    G, pos = cohort_small
+    pairs = O.all_pairs(G.shape[0]) + [(3, 3)] + O.all_pairs(G.shape[0])[:20]  # 87 pairs: one partial tile
+    T = _tables(flow_field_path, 1.25e-8, 1e-8)
+    ctx = core.FlowContext(G, pos, 1e4, 1.25e-8, 1e-8, flow_field_path, 0)
+    ctx.set_tuning(checkpoint_k=8, pairs_per_lane=4, schedule=schedule, min_waves_per_launch=1)
+    _case(f"four pairs per lane {schedule}: blockwise")
+    ref = O.fb_blockwise_f32(G, pos, pairs, T, 1e4, 200, 64)["mean"]
+    got = ctx.run_fb_blockwise(pairs, core_block_sites=200, flank_sites=64, pair_batch_size=-1, max_streams=2)
+    assert _bits_equal(got["mean"], ref)
+    _case(f"four pairs per lane {schedule}: u8 codes")
+    codes = ctx.run_fb_codes(pairs, bits=8, core_block_sites=200, flank_sites=64)["codes"]
+    d = np.abs(codes.astype(np.int32) - O.log_codes(ref, 8, 1e4).astype(np.int32))
+    assert d.max() <= 1 and np.count_nonzero(d) <= 1e-3 * d.size
+    _case(f"four pairs per lane {schedule}: summary")
+    s = ctx.run_fb_summary(pairs, core_block_sites=200, flank_sites=64)
+    want = O.site_summary(ref)
+    assert _bits_equal(s["site_mean"], want["site_mean"])
+    _case(f"four pairs per lane {schedule}: ragged batch")
+    jobs = _ragged_jobs(G.shape[0], G.shape[1], np.random.default_rng(41))
+    for j, a in zip(jobs, batch.decode_batch(ctx, jobs, mean_only=True)):
+        pj = [tuple(map(int, p)) for p in j.pair_array()]
+        if j.n_sites == 0 or not pj:
+            continue
+        core_sites = j.core_block_sites if j.core_block_sites > 0 else j.n_sites
+        flank = j.flank_sites if j.core_block_sites > 0 else 0
+        r = O.fb_blockwise_f32(np.ascontiguousarray(G[:, j.site_lo:j.site_hi]),
+                               np.ascontiguousarray(pos[j.site_lo:j.site_hi]), pj, T, 1e4, core_sites, flank)
+        assert _bits_equal(a["mean"], r["mean"])
 
 
 @pytest.mark.parametrize("schedule", ["static", "persistent"])
