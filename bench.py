@@ -58,8 +58,11 @@ REFERENCE_E2E_S = 23.659  # benchmarks/pairwise_scaling/results.csv:22 (1x B200,
 # counter summary under profiles/ gives the commands. Any other kernel
 # source reports traffic null.
 MEASURED_TRAFFIC = {
-    "kernel_sha256": None,
-    "entries": {},
+    "kernel_sha256": "557c51403492c4b1263a0b5756a3aa010f6d7aae6677b3b378660cf041c442b2",
+    "entries": {
+        "chromosome/K8/np1/persistent/xcd0": 169.89e9,
+        "chromosome/K8/np1/static/xcd0": 170.42e9,
+    },
 }
 
 WORKLOADS = {
