@@ -131,3 +131,46 @@ def test_reserved_workspace_keeps_its_checkpoint_buffer(flow_field_path):
     res = st.results()
     for k in (0, 1, 29):
         assert _eq_bits(res[k]["mean"], mix[k]["mean"]), k
+
+
+@pytest.mark.parametrize("k,schedule", [(1, "persistent"), (8, "persistent"), (8, "static")])
+def test_workspace_reused_across_streams(flow_field_path, k, schedule):
+    """One workspace, launches with different metadata on two streams and
+    the null stream in turn, queued without host waits between them: each
+    launch first waits (on the host) for the workspace's previous launch
+    before its metadata upload and queue-counter reset, so no kernel reads
+    metadata or counters another launch is rewriting. Every job's rows equal
+    the oracle's."""
+    from gamma_smc_cu_amd.device import Stream
+
+    G, pos = synthetic.cohort(16, 1500, 450_000, 9)
+    mu, rho = O.estimate_scaled_params(G, pos, 1.25e-8, 1e-8, 1e4)
+    ctx = gamma_smc_cu_amd.FlowContext(G, pos, 1e4, mu, rho, flow_field_path, 0)
+    ctx.set_tuning(checkpoint_k=k, schedule=schedule)
+    rng = np.random.default_rng(3)
+    allp = O.all_pairs(16)
+    jobs = []
+    for _ in range(9):
+        lo = int(rng.integers(0, 1000))
+        hi = int(rng.integers(lo + 50, 1500))
+        sel = rng.choice(len(allp), size=int(rng.integers(5, 90)), replace=False)
+        jobs.append(Job(lo, hi, [allp[s] for s in sel], core_block_sites=int(rng.choice([0, 200])), flank_sites=40))
+    st = Staged(jobs)
+    ws = Workspace(ctx)
+    s1, s2 = Stream(), Stream()
+    streams = [s1, None, s2]
+    _case(f"workspace across streams K={k} {schedule}")
+    for i in range(len(jobs)):
+        arr, n = st.structs([i])
+        ws.launch(arr, n, st.flags, streams[i % 3])
+    ws.wait()
+    s1.synchronize()
+    s2.synchronize()
+    T = O.tables_f32(flow_field_path, 1000, 1e4, mu, rho)
+    for j, r in zip(jobs, st.results()):
+        pairs = [tuple(map(int, p)) for p in j.pair_array()]
+        core_sites = j.core_block_sites if j.core_block_sites > 0 else j.n_sites
+        flank = j.flank_sites if j.core_block_sites > 0 else 0
+        ref = O.fb_blockwise_f32(np.ascontiguousarray(G[:, j.site_lo:j.site_hi]), pos[j.site_lo:j.site_hi], pairs, T,
+                                 1e4, core_sites, flank)
+        assert _eq_bits(r["mean"], ref["mean"])
