@@ -340,8 +340,8 @@ def run_decode_workload(args, name, compact=False):
         "bound": "ta",
         "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
         "traffic": measured_traffic(name, tuning),
-        "kernel": ("fb_fwd_items + fb_bwd_items" if tuning["schedule"] == "static" else "fb_fwd_queue + fb_bwd_queue")
-        if K != 1 else "stored-state kernels",
+        "kernel": {"static": "fb_fwd_items + fb_bwd_items", "persistent": "fb_fwd_queue + fb_bwd_queue",
+                   "dynamic": "fb_fwd_grab + fb_bwd_grab"}[tuning["schedule"]] if K != 1 else "stored-state kernels",
         "algorithmic_bytes_per_launch": B,
         "frac_output_floor": floor_bytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
         "output_floor_bytes_per_launch": floor_bytes,
@@ -720,7 +720,7 @@ def main():
                     help="auto: sweep launch shapes first and keep the fastest (results are identical)")
     ap.add_argument("--checkpoint-k", type=int, default=16)
     ap.add_argument("--pairs-per-lane", type=int, default=1)
-    ap.add_argument("--schedule", default="static", choices=("static", "persistent"))
+    ap.add_argument("--schedule", default="static", choices=("static", "persistent", "dynamic"))
     ap.add_argument("--xcd-order", type=int, default=0)
     ap.add_argument("--cpu-pairs", type=int, default=1000)
     ap.add_argument("--no-cpu-baseline", action="store_true")

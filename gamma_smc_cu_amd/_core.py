@@ -355,10 +355,14 @@ class FlowContext:
     def set_tuning(self, checkpoint_k=16, pairs_per_lane=1, schedule="static", xcd_order=False,
                    min_waves_per_launch=8192):
         """Launch shape (results are identical under every setting).
+        ``schedule``: ``"static"`` (workgroup g decodes item g),
+        ``"persistent"`` (resident waves drain a device queue) or
+        ``"dynamic"`` (one workgroup per item, each taking the next item of
+        the queue).
         ``checkpoint_k=1`` stores the forward state at every core site
         (8 bytes per pair and site of device memory) instead of
         recomputing it in the backward kernel."""
-        sched = {"static": 0, "persistent": 1}[schedule]
+        sched = {"static": 0, "persistent": 1, "dynamic": 2}[schedule]
         check(lib().gsmc_context_set_tuning(self._h, int(checkpoint_k), int(pairs_per_lane), sched,
                                             int(bool(xcd_order)), int(min_waves_per_launch)))
 

@@ -15,8 +15,9 @@ fastest. Candidates sweep:
   pairs per 64-lane workgroup (more pairs give every phase more independent
   lookup chains per lane, at the cost of registers and, at 4, of the LDS
   that holds the recomputed forward states; 4 is built for checkpoint_k 8);
-* ``schedule`` -- a static grid of work items, ordered per XCD or not, or
-  persistent waves draining a device queue.
+* ``schedule`` -- a static grid of work items, ordered per XCD or not,
+  persistent waves draining a device queue, or a dynamic grid (one
+  workgroup per item, each taking the next item of the queue).
 
 The core block size is not a tuning knob: block windows change the
 posterior (each window restarts from the prior), so they stay what the
@@ -31,9 +32,10 @@ import time
 CANDIDATES = [
     dict(checkpoint_k=k, pairs_per_lane=np_, schedule=sch, xcd_order=xcd)
     for k, np_, (sch, xcd) in itertools.product(
-        (8, 16), (1, 2), (("static", False), ("static", True), ("persistent", False)))
+        (8, 16), (1, 2), (("static", False), ("static", True), ("persistent", False), ("dynamic", False)))
 ] + [dict(checkpoint_k=8, pairs_per_lane=4, schedule=sch, xcd_order=False) for sch in ("static", "persistent")] \
-  + [dict(checkpoint_k=1, pairs_per_lane=np_, schedule="static", xcd_order=False) for np_ in (1, 2)] \
+  + [dict(checkpoint_k=1, pairs_per_lane=np_, schedule=sch, xcd_order=False)
+     for np_ in (1, 2) for sch in ("static", "dynamic")] \
   + [dict(checkpoint_k=1, pairs_per_lane=1, schedule="persistent", xcd_order=False)]
 
 

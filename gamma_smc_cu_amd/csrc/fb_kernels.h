@@ -1025,6 +1025,28 @@ __device__ __forceinline__ int next_item(int* counter, int lane)
     return __builtin_amdgcn_readfirstlane(__shfl(item, 0));
 }
 
+// Dynamic schedule: one workgroup per item, as the static grid, but each
+// workgroup decodes the next item of the device counter (items in their
+// longest-first order) instead of item blockIdx.x, so items are handed to
+// whichever XCD frees a slot first; no item loop, so no values live across
+// items.
+template <int K, int NP, bool ST>
+__global__ __launch_bounds__(64) void fb_fwd_grab(FbArgs a)
+{
+    const int item = next_item(a.queue, threadIdx.x);
+    if (item < a.n_items) forward_item<K, NP, ST>(a, item, threadIdx.x);
+}
+
+template <int K, int NP, bool CI, bool POST, bool SUM, bool ST>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kBwdWaves<K, NP, ST>))) void fb_bwd_grab(FbArgs a)
+{
+    __shared__ float red[kBwdLds<K, CI, POST, SUM>];
+    __shared__ float2 states[kStateLds<K, NP, ST>];
+    const int item = next_item(a.queue2, threadIdx.x);
+    if (item < a.n_items)
+        backward_item<K, NP, CI, POST, SUM, ST, kStatesInLds<K, ST>>(a, item, threadIdx.x, red, -1, states);
+}
+
 template <int K, int NP, bool ST>
 __global__ __launch_bounds__(64) void fb_fwd_queue(FbArgs a)
 {

@@ -792,6 +792,8 @@ struct KernelPair {
     const void* bwd_items;
     const void* fwd_queue;
     const void* bwd_queue;
+    const void* fwd_grab;
+    const void* bwd_grab;
 };
 
 template <int K, int NP, bool C, bool Q, bool SUM = false, bool ST = false> KernelPair kernels_of()
@@ -799,7 +801,9 @@ template <int K, int NP, bool C, bool Q, bool SUM = false, bool ST = false> Kern
     return {reinterpret_cast<const void*>(&fb_fwd_items<K, NP, ST>),
             reinterpret_cast<const void*>(&fb_bwd_items<K, NP, C, Q, SUM, ST>),
             reinterpret_cast<const void*>(&fb_fwd_queue<K, NP, ST>),
-            reinterpret_cast<const void*>(&fb_bwd_queue<K, NP, C, Q, SUM, ST>)};
+            reinterpret_cast<const void*>(&fb_bwd_queue<K, NP, C, Q, SUM, ST>),
+            reinterpret_cast<const void*>(&fb_fwd_grab<K, NP, ST>),
+            reinterpret_cast<const void*>(&fb_bwd_grab<K, NP, C, Q, SUM, ST>)};
 }
 
 // The kernels for the pairs per lane launch_np plans with (the tile size of
@@ -823,7 +827,9 @@ template <int NP, bool SUM> KernelPair stored_kernels()
     return {reinterpret_cast<const void*>(&fb_fwd_items<16, NP, true>),
             reinterpret_cast<const void*>(&fb_bwd_items<kStoredBwdK, NP, false, false, SUM, true>),
             reinterpret_cast<const void*>(&fb_fwd_queue<16, NP, true>),
-            reinterpret_cast<const void*>(&fb_bwd_queue<kStoredBwdK, NP, false, false, SUM, true>)};
+            reinterpret_cast<const void*>(&fb_bwd_queue<kStoredBwdK, NP, false, false, SUM, true>),
+            reinterpret_cast<const void*>(&fb_fwd_grab<16, NP, true>),
+            reinterpret_cast<const void*>(&fb_bwd_grab<kStoredBwdK, NP, false, false, SUM, true>)};
 }
 
 KernelPair kernels_for(const Tuning& tu, int flags)
@@ -928,6 +934,11 @@ void enqueue_kernels(gsmc_context* ctx, const Plan& pl, const FbArgs& a, int fla
                    stream);
             launch(kp.bwd_queue, (unsigned)std::max<int64_t>(1, std::min(resident(ctx, kp.bwd_queue), ph.count)), pa,
                    stream);
+        } else if (tu.schedule == 2) {
+            // one workgroup per item, each taking the next item of the counter
+            hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(1), 0, stream, a.queue);
+            launch(kp.fwd_grab, (unsigned)ph.count, pa, stream);
+            launch(kp.bwd_grab, (unsigned)ph.count, pa, stream);
         } else {
             launch(kp.fwd_items, (unsigned)ph.count, pa, stream);
             launch(kp.bwd_items, (unsigned)ph.count, pa, stream);
@@ -1340,7 +1351,7 @@ int gsmc_context_set_tuning(gsmc_context* ctx, int k, int pairs_per_lane, int sc
         require(k == 1 || k == 8 || k == 16, "checkpoint_k must be 1 (stored forward states), 8 or 16");
         require(pairs_per_lane == 1 || pairs_per_lane == 2 || pairs_per_lane == 4, "pairs_per_lane must be 1, 2 or 4");
         require(pairs_per_lane != 4 || k == 8, "pairs_per_lane 4 needs checkpoint_k 8");
-        require(schedule == 0 || schedule == 1, "schedule must be 0 (static) or 1 (persistent)");
+        require(schedule >= 0 && schedule <= 2, "schedule must be 0 (static), 1 (persistent) or 2 (dynamic)");
         require(min_waves >= 1, "min_waves_per_launch must be positive");
         std::lock_guard<std::mutex> lk(ctx->mu);
         ctx->tune.K = k;

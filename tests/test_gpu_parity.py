@@ -166,6 +166,7 @@ SCHEDULES = [dict(checkpoint_k=k, pairs_per_lane=np_, schedule=sch, xcd_order=xc
              for sch, xcd in (("static", True), ("static", False), ("persistent", False))]
 SCHEDULES += [dict(checkpoint_k=1, pairs_per_lane=1, schedule=sch, xcd_order=xcd)
               for sch, xcd in (("static", True), ("persistent", False))]
+SCHEDULES += [dict(checkpoint_k=k, pairs_per_lane=1, schedule="dynamic", xcd_order=False) for k in (8, 16, 1)]
 
 
 def test_launch_shapes_are_bit_identical(core, cohort_small, flow_field_path):
@@ -200,7 +201,8 @@ def _ragged_jobs(n_haps, S, rng):
     return jobs
 
 
-@pytest.mark.parametrize("tu", [SCHEDULES[0], SCHEDULES[4], SCHEDULES[8], SCHEDULES[-1]])
+@pytest.mark.parametrize("tu", [SCHEDULES[0], SCHEDULES[4], SCHEDULES[8], SCHEDULES[13], SCHEDULES[14],
+                                SCHEDULES[16]])
 def test_batch_matches_alone_and_oracle(core, cohort_small, flow_field_path, tu):
     from gamma_smc_cu_amd import batch
 
@@ -232,7 +234,8 @@ def test_batch_matches_alone_and_oracle(core, cohort_small, flow_field_path, tu)
 
 @pytest.mark.parametrize("schedule,budget,np_", [("static", 0, 1), ("persistent", 0, 1), ("static", 400_000, 1),
                                                  ("persistent", 800_000, 1), ("static", 0, 2),
-                                                 ("static", 800_000, 2)])
+                                                 ("static", 800_000, 2), ("dynamic", 0, 1),
+                                                 ("dynamic", 800_000, 2)])
 def test_stored_forward_states_bit_exact(core, cohort_small, flow_field_path, schedule, budget, np_):
     """checkpoint_k=1: each wave's forward walk stores the state after
     every core site in the wave's slot and its backward walk reads it back
