@@ -102,12 +102,15 @@ def decode_lookups(P, S, core, flank, K):
     """Table lookups of one decode: per block, the forward walk over
     [ps, ce) and the backward walk over the right flank (pe - ps in all),
     then per core site the backward step and, with checkpoints, the
-    recomputed forward step (stored forward states, K = 1, skip it).
-    Gaps longer than the table depth add stretch lookups; the workloads
-    here have none."""
+    recomputed forward step of every site but the first of each K-site
+    window, whose state is the checkpoint (stored forward states, K = 1,
+    recompute none). Gaps longer than the table depth add stretch lookups;
+    the workloads here have none."""
     per = 0
     for cs, ce, ps, pe in block_windows(S, core, flank):
-        per += (pe - ps) + (ce - cs) * (1 if K == 1 else 2)
+        n = ce - cs
+        recompute = 0 if K == 1 else n - (n + K - 1) // K
+        per += (pe - ps) + n + recompute
     return P * per
 
 

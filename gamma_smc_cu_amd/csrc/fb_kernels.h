@@ -640,10 +640,14 @@ __device__ __forceinline__ void window_step(const FbArgs& a, const JobDev& j, co
                     }
                     continue;
                 }
-                if (FAST)
-                    lookup(layer_of(a.t.fwd, nxt.d[i]), het_slab(nxt.bits[q], i), m[q], c[q]);
-                else
-                    fwd_segment(a.t, nxt.d[i], het_slab(nxt.bits[q], i), m[q], c[q]);
+                // the next window's checkpoint is already the state of its
+                // first site (i = 0)
+                if (i > 0) {
+                    if (FAST)
+                        lookup(layer_of(a.t.fwd, nxt.d[i]), het_slab(nxt.bits[q], i), m[q], c[q]);
+                    else
+                        fwd_segment(a.t, nxt.d[i], het_slab(nxt.bits[q], i), m[q], c[q]);
+                }
                 F.set(q, slot, m[q], c[q]);  // next window: site ws - K + i, in the slot just freed
             }
         }
@@ -736,8 +740,8 @@ template <int NP> __device__ __forceinline__ float2* yb_at(const FbArgs& a, int 
 }
 
 // First half of an item (64 NP pairs x one block window): the forward walk
-// over [ps, ce) storing the state at the start of every core window (ST:
-// the state after every core site), and the backward walk over the right
+// over [ps, ce) storing the state after the first site of every core window
+// (ST: after every core site), and the backward walk over the right
 // flank [ce, pe), whose final state is the backward state entering the core.
 template <int K, int NP, bool ST>
 __device__ __forceinline__ void forward_item(const FbArgs& a, int item, int lane, int slot = -1)
@@ -760,25 +764,24 @@ __device__ __forceinline__ void forward_item(const FbArgs& a, int item, int lane
     }
     HetCache<NP> hf, hb;
     float ckm[NP], ckc[NP];  // the even window's checkpoint until its odd partner's
+    // A core window's checkpoint is the state after its first site, so the
+    // backward half recomputes K - 1 sites per window, not K. Checkpoints of
+    // windows 2r and 2r + 1 share 16-byte record r; a last even window fills
+    // both halves.
+    auto checkpoint = [&](int64_t w, int q) {
+        if (w & 1) {
+            put_rec(rec_at(L, w >> 1, q), ckm[q], ckc[q], m[q], c[q]);
+        } else if (w + 1 == b.nwin) {
+            put_rec(rec_at(L, w >> 1, q), m[q], c[q], m[q], c[q]);
+        } else {
+            ckm[q] = m[q];
+            ckc[q] = c[q];
+        }
+    };
     for (int64_t s = ps; s < ce;) {
         const int64_t e = (s < cs) ? min(cs, s + K) : min(ce, s + K);
         const bool core = s >= cs;  // windows are all flank or all core
-        if (!ST && core) {
-            // checkpoints of windows 2r and 2r + 1 share 16-byte record r; a
-            // last even window fills both halves
-            const int64_t w = (s - cs) / K;
-#pragma unroll
-            for (int q = 0; q < NP; ++q) {
-                if (w & 1) {
-                    put_rec(rec_at(L, w >> 1, q), ckm[q], ckc[q], m[q], c[q]);
-                } else if (w + 1 == b.nwin) {
-                    put_rec(rec_at(L, w >> 1, q), m[q], c[q], m[q], c[q]);
-                } else {
-                    ckm[q] = m[q];
-                    ckc[q] = c[q];
-                }
-            }
-        }
+        const int64_t w = (s - cs) / K;
         Window<K, NP> win;
         load_window(a, L, s, win, hf);
         if (s == ps) win.d[0] = b.sd_fwd;
@@ -792,6 +795,7 @@ __device__ __forceinline__ void forward_item(const FbArgs& a, int item, int lane
 #pragma unroll
                 for (int q = 0; q < NP; ++q) {
                     lookup(layer_of(a.t.fwd, win.d[k]), het_slab(win.bits[q], k), m[q], c[q]);
+                    if (!ST && core && k == 0) checkpoint(w, q);
                     if (ST && core) {
                         if (k & 1) {
                             put_rec(rec_at(L, (s - cs + k) >> 1, q), pm[q], pc[q], m[q], c[q]);
@@ -808,6 +812,7 @@ __device__ __forceinline__ void forward_item(const FbArgs& a, int item, int lane
 #pragma unroll
                     for (int q = 0; q < NP; ++q) {
                         fwd_segment(a.t, win.d[k], het_slab(win.bits[q], k), m[q], c[q]);
+                        if (!ST && core && k == 0) checkpoint(w, q);
                         if (ST && core) {
                             if (k & 1) {
                                 put_rec(rec_at(L, (s - cs + k) >> 1, q), pm[q], pc[q], m[q], c[q]);
@@ -894,8 +899,7 @@ __device__ __forceinline__ void backward_item(const FbArgs& a, int item, int lan
             }
         } else {
             Window<K, NP> win;
-            load_window(a, L, ws, win, hn);
-            if (ws == ps) win.d[0] = b.sd_fwd;
+            load_window(a, L, ws, win, hn);  // descriptor 0 unused: site ws is the checkpoint
 #pragma unroll
             for (int q = 0; q < NP; ++q) {
                 // record w >> 1 holds the checkpoints of windows w & ~1 and w | 1
@@ -904,9 +908,10 @@ __device__ __forceinline__ void backward_item(const FbArgs& a, int item, int lan
                 c[q] = (w & 1) ? r.w : r.y;
                 ckm[q] = r.x;  // window w - 1's checkpoint when w is odd
                 ckc[q] = r.y;
+                F.set(q, 0, m[q], c[q]);  // the checkpoint is the state of site ws
             }
 #pragma unroll
-            for (int k = 0; k < K; ++k) {
+            for (int k = 1; k < K; ++k) {
                 if (k < cnt) {
 #pragma unroll
                     for (int q = 0; q < NP; ++q) {
@@ -926,7 +931,7 @@ __device__ __forceinline__ void backward_item(const FbArgs& a, int item, int lan
         if (ws == ps) cur.d[0] = b.sd_bwd;
         if (has_next && !ST) {
             load_window(a, L, ws - K, nxt, hn);
-            if (ws - K == ps) nxt.d[0] = b.sd_fwd;
+            nxt.d[0] = 0;  // unused: site ws - K is the checkpoint (and no long gap)
 #pragma unroll
             for (int q = 0; q < NP; ++q) {
                 if ((w - 1) & 1) {
@@ -1018,11 +1023,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kBwdWaves<K,
         backward_item<K, NP, CI, POST, SUM, ST, kStatesInLds<K, ST>>(a, item, threadIdx.x, red, -1, states);
 }
 
+// Lane 0 takes the next item of the counter; every caller reaches this with
+// all 64 lanes active (uniform control flow), so the first active lane is
+// lane 0 and readfirstlane broadcasts its value without a lane permute (the
+// permute kept extra registers live and cost the backward kernels a wave).
 __device__ __forceinline__ int next_item(int* counter, int lane)
 {
     int item = 0;
     if (lane == 0) item = atomicAdd(counter, 1);
-    return __builtin_amdgcn_readfirstlane(__shfl(item, 0));
+    return __builtin_amdgcn_readfirstlane(item);
 }
 
 // Dynamic schedule: one workgroup per item, as the static grid, but each
