@@ -58,10 +58,11 @@ REFERENCE_E2E_S = 23.659  # benchmarks/pairwise_scaling/results.csv:22 (1x B200,
 # counter summary under profiles/ gives the commands. Any other kernel
 # source reports traffic null.
 MEASURED_TRAFFIC = {
-    "kernel_sha256": "557c51403492c4b1263a0b5756a3aa010f6d7aae6677b3b378660cf041c442b2",
+    "kernel_sha256": "c7d179e9f295061bcc82489b4c02b08e1991b652264e540113e6c1cf8846b30b",
     "entries": {
-        "chromosome/K8/np1/persistent/xcd0": 169.89e9,
-        "chromosome/K8/np1/static/xcd0": 170.42e9,
+        "chromosome/K8/np1/dynamic/xcd0": 169.41e9,
+        "chromosome/K8/np1/persistent/xcd0": 168.88e9,
+        "chromosome/K8/np1/static/xcd0": 167.52e9,
     },
 }
 
