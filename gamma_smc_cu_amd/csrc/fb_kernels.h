@@ -31,20 +31,15 @@
 namespace gsmc {
 
 constexpr int kGridNodes = 2550;
-// Table slabs are stored by cell row pairs: entry (r, q), r < 50, q < 50,
-// is the 16-byte {m, c at node (r, q); m, c at node (r + 1, q)}, so the four
-// corners of cell (r, q) are the 32 contiguous bytes of entries (r, q) and
-// (r, q + 1).
-// Each row of entries is laid out in 128-byte chunks of eight entries,
-// chunk t holding entries 7t .. 7t + 7 (consecutive chunks overlap by one),
-// so the 32 bytes of any cell lie inside one aligned 128-byte cache line:
-// 7 chunks, 896 bytes per row.
+// Table slabs are stored by cell: cell (r, q), r < 50, q < 49, is 32 bytes
+// at (49 r + q) * 32, its entries (r, q) and (r, q + 1), each the 16-byte
+// {m, c at node (r, q'); m, c at node (r + 1, q')}. A cell is the four
+// corners one lookup reads, 32-byte aligned, so it never crosses a 128-byte
+// cache line, and its offset is one multiply-add of the cell indices.
 constexpr uint32_t kPairRows = 50;
-constexpr uint32_t kChunkEntries = 8;
-constexpr uint32_t kChunkStride = kChunkEntries - 1;
-constexpr uint32_t kRowChunks = 7;
-constexpr uint32_t kRowBytes = kRowChunks * kChunkEntries * 16;
-constexpr uint32_t kSlabBytes = kPairRows * kRowBytes;  // one slab: 44,800 bytes
+constexpr uint32_t kRowCells = 49;
+constexpr uint32_t kRowBytes = kRowCells * 32;
+constexpr uint32_t kSlabBytes = kPairRows * kRowBytes;  // one slab: 78,400 bytes
 constexpr uint32_t kLayerBytes = 2 * kSlabBytes;        // hom slab then het slab
 constexpr uint32_t kDescLayerBits = 13;              // site descriptor: layer | chunks << 13
 constexpr uint32_t kDescLayerMask = (1u << kDescLayerBits) - 1;
@@ -194,10 +189,8 @@ __device__ __forceinline__ void lookup(const char* __restrict__ layer, uint32_t 
     const float lc = fminf(floorf(fc), 48.0f);
     const float wm = fm - lm;
     const float wc = fc - lc;
-    // entry (r, q) of the first corner pair at 16-byte unit r * 56 + q + q / 7
-    // (chunk q / 7, position q % 7 in it); all exact small integers in float
-    const float q3 = floorf(lc * 0.142857149f);  // lc div 7 for lc < 49 (the factor rounds above 1/7)
-    const uint32_t off = ((uint32_t)fmaf(lm, (float)(kRowBytes / 16), lc + q3) << 4) + slab;
+    // cell (lm, lc) at 32-byte unit 49 lm + lc, an exact small integer in float
+    const uint32_t off = ((uint32_t)fmaf(lm, (float)kRowCells, lc) << 5) + slab;
     const bool upper = (__lane_id() & 2u) != 0;
     // first gather: lanes 0, 1 their entry (r, q + 1); lanes 2, 3 entry (r, q) of lanes 0, 1
     const uint32_t a1 = quad_perm<kQuad0101>(off) + (upper ? 0u : 16u);

@@ -217,18 +217,17 @@ std::shared_ptr<DeviceTables> get_tables(int device, const char* path, int N, do
     dt->bytes = 2 * layers + kSlabBytes;
     HIP_OK(hipMalloc(&dt->base, dt->bytes));
     {
-        // Node-major (m, c) slabs -> row-pair entries (see kSlabBytes).
+        // Node-major (m, c) slabs -> 32-byte cells (see kSlabBytes).
         std::vector<float> dev(dt->bytes / sizeof(float));
         auto slab = [&](const float* nodes, float* out) {
             for (uint32_t r = 0; r < kPairRows; ++r)
-                for (uint32_t t = 0; t < kRowChunks; ++t)
-                    for (uint32_t i = 0; i < kChunkEntries; ++i) {
-                        const uint32_t q = std::min<uint32_t>(kChunkStride * t + i, 49);  // entries past 49 are never read
-                        float* e = out + ((size_t)r * kRowBytes + (t * kChunkEntries + i) * 16) / sizeof(float);
-                        e[0] = nodes[2 * (r * 50 + q)];
-                        e[1] = nodes[2 * (r * 50 + q) + 1];
-                        e[2] = nodes[2 * ((r + 1) * 50 + q)];
-                        e[3] = nodes[2 * ((r + 1) * 50 + q) + 1];
+                for (uint32_t q = 0; q < kRowCells; ++q)
+                    for (uint32_t h = 0; h < 2; ++h) {  // entries (r, q) and (r, q + 1)
+                        float* e = out + ((size_t)r * kRowBytes + q * 32 + h * 16) / sizeof(float);
+                        e[0] = nodes[2 * (r * 50 + q + h)];
+                        e[1] = nodes[2 * (r * 50 + q + h) + 1];
+                        e[2] = nodes[2 * ((r + 1) * 50 + q + h)];
+                        e[3] = nodes[2 * ((r + 1) * 50 + q + h) + 1];
                     }
         };
         const size_t slab_f = kSlabBytes / sizeof(float), node_f = (size_t)kGridNodes * 2;
