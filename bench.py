@@ -58,11 +58,11 @@ REFERENCE_E2E_S = 23.659  # benchmarks/pairwise_scaling/results.csv:22 (1x B200,
 # counter summary under profiles/ gives the commands. Any other kernel
 # source reports traffic null.
 MEASURED_TRAFFIC = {
-    "kernel_sha256": "c7d179e9f295061bcc82489b4c02b08e1991b652264e540113e6c1cf8846b30b",
+    "kernel_sha256": "ce0011d09d4fc04a7c15c7961770907f5bc119754767235ac2454a83264aed95",
     "entries": {
-        "chromosome/K8/np1/dynamic/xcd0": 169.41e9,
-        "chromosome/K8/np1/persistent/xcd0": 168.88e9,
-        "chromosome/K8/np1/static/xcd0": 167.52e9,
+        "chromosome/K8/np1/dynamic/xcd0": 177.77e9,
+        "chromosome/K8/np1/persistent/xcd0": 177.90e9,
+        "chromosome/K8/np1/static/xcd0": 180.74e9,
     },
 }
 
