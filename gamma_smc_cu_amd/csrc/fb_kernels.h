@@ -31,15 +31,18 @@
 namespace gsmc {
 
 constexpr int kGridNodes = 2550;
-// Table slabs are stored by cell: cell (r, q), r < 50, q < 49, is 32 bytes
-// at (49 r + q) * 32, its entries (r, q) and (r, q + 1), each the 16-byte
-// {m, c at node (r, q'); m, c at node (r + 1, q')}. A cell is the four
-// corners one lookup reads, 32-byte aligned, so it never crosses a 128-byte
-// cache line, and its offset is one multiply-add of the cell indices.
-constexpr uint32_t kPairRows = 50;
-constexpr uint32_t kRowCells = 49;
+// Table slabs are stored by cell: cell (r, q), r <= 50, q <= 49, is 32
+// bytes at (50 r + q) * 32, its entries (r, q) and (r, q + 1), each the
+// 16-byte {m, c at node (r, q'); m, c at node (r + 1, q')}. A cell is the
+// four corners one lookup reads, 32-byte aligned, so it never crosses a
+// 128-byte cache line, and its offset is one multiply-add of the cell
+// indices. The last row and column of cells (r = 50, q = 49) read node row
+// 51 as a copy of row 49 and node column 50 as a copy of column 48; see
+// lookup() for why that reproduces the reference at the grid's upper edges.
+constexpr uint32_t kPairRows = 51;
+constexpr uint32_t kRowCells = 50;
 constexpr uint32_t kRowBytes = kRowCells * 32;
-constexpr uint32_t kSlabBytes = kPairRows * kRowBytes;  // one slab: 78,400 bytes
+constexpr uint32_t kSlabBytes = kPairRows * kRowBytes;  // one slab: 81,600 bytes
 constexpr uint32_t kLayerBytes = 2 * kSlabBytes;        // hom slab then het slab
 constexpr uint32_t kDescLayerBits = 13;              // site descriptor: layer | chunks << 13
 constexpr uint32_t kDescLayerMask = (1u << kDescLayerBits) - 1;
@@ -165,11 +168,15 @@ constexpr int kQuad2301 = 0x4E;  // 2, 3, 0, 1
 // wave must call it together.
 //
 // Same values as gamma_smc_flow.cu:408-430 in fewer instructions: the clamp
-// is one med3 (identical for any non-NaN index), the cell corner is
-// floor(index) capped at the last cell (identical to truncating a
-// non-negative index and decrementing 50 -> 49 / 49 -> 48), and the byte
-// offset of the corner entry is formed in float, exact for these small
-// integers.
+// is one med3 (identical for any non-NaN index) and the cell corner is
+// floor(index) (truncation of a non-negative index), formed in float, exact
+// for these small integers. The reference moves an index at the upper edge
+// (50 for the mean, 49 for the CV) down one cell, with weight 1: corner row
+// 49, value fma(1, A50, 0 * A49) where A_r interpolates node row r. Here
+// that index stays in edge cell 50 with weight 0, whose node row 51 is a
+// copy of row 49: fma(0, A49, 1 * A50). Both fused multiply-adds add the
+// same two exact terms, A50 and a zero signed like A49, so the results are
+// identical, zeros' signs and NaNs included; likewise for the CV column.
 //
 // Loads: the texture path handles a 16-byte gather four lanes at a time and
 // pays one cache access per distinct line in each such quad. Each lane needs
@@ -185,11 +192,11 @@ __device__ __forceinline__ void lookup(const char* __restrict__ layer, uint32_t 
     float fc = (c - -2.0f) * kCvStepInv;
     fm = __builtin_amdgcn_fmed3f(fm, 0.0f, 50.0f);
     fc = __builtin_amdgcn_fmed3f(fc, 0.0f, 49.0f);
-    const float lm = fminf(floorf(fm), 49.0f);
-    const float lc = fminf(floorf(fc), 48.0f);
+    const float lm = floorf(fm);
+    const float lc = floorf(fc);
     const float wm = fm - lm;
     const float wc = fc - lc;
-    // cell (lm, lc) at 32-byte unit 49 lm + lc, an exact small integer in float
+    // cell (lm, lc) at 32-byte unit 50 lm + lc, an exact small integer in float
     const uint32_t off = ((uint32_t)fmaf(lm, (float)kRowCells, lc) << 5) + slab;
     const bool upper = (__lane_id() & 2u) != 0;
     // first gather: lanes 0, 1 their entry (r, q + 1); lanes 2, 3 entry (r, q) of lanes 0, 1
@@ -230,11 +237,12 @@ __device__ __forceinline__ void bwd_segment(const DevTables& t, uint32_t d, uint
 }
 
 // Schraudolph-style 10^x as upstream gamma_smc reconstructs it
-// (gamma_smc_flow.cu:115-123); the multiply-add is fused as nvcc fuses it.
-__device__ __forceinline__ float pow10_fast(float x)
+// (gamma_smc_flow.cu:115-123), bits = (int)(12102203.16 * (x * 2.30258509)
+// + 1064872507.15) with the multiply-add fused as nvcc fuses it, given the
+// product xk = x * 2.30258509299f.
+__device__ __forceinline__ float pow10_fast_of_product(float xk)
 {
-    const int bits = (int)(fmaf(12102203.161561485f, x * 2.30258509299f, 1064872507.1541044f));
-    return __int_as_float(bits);
+    return __int_as_float((int)(fmaf(12102203.161561485f, xk, 1064872507.1541044f)));
 }
 
 // Het bits of the 64 sites starting at s0 (bit k = site s0 + k).
@@ -295,13 +303,17 @@ __device__ __forceinline__ void store_stream(float* row, uint32_t col, float v)
     __builtin_nontemporal_store(v, global_at<float>(row, col));
 }
 
+// Posterior mean of the forward (fm, fc) and backward (ym, yc) states
+// (gamma_smc_flow.cu:708-723): alpha = 10^(-2 fc) + 10^(-2 yc) - 1, beta =
+// 10^(-2 fc - fm) + 10^(-2 yc - ym) - 1. The reference forms -2 c first;
+// that product is exact, so (-2 c) * k equals c * (-2 k) and (-2 c) - m
+// equals fma(-2, c, -m), bit for bit, one instruction fewer each.
 __device__ __forceinline__ float combine_mean(float fm, float fc, float ym, float yc, float unscale, float& as,
                                               float& bs)
 {
-    const float lb = -2.0f * yc;
-    const float ba = pow10_fast(lb), bb = pow10_fast(lb - ym);
-    const float lf = -2.0f * fc;
-    const float fa = pow10_fast(lf), fb = pow10_fast(lf - fm);
+    constexpr float k = 2.30258509299f, m2k = -2.0f * k;
+    const float ba = pow10_fast_of_product(yc * m2k), bb = pow10_fast_of_product(fmaf(-2.0f, yc, -ym) * k);
+    const float fa = pow10_fast_of_product(fc * m2k), fb = pow10_fast_of_product(fmaf(-2.0f, fc, -fm) * k);
     as = fa + ba - 1.0f;
     bs = fb + bb - 1.0f;
     return (as / fmaxf(bs, 1e-10f)) * unscale;

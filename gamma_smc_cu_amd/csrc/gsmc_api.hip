@@ -217,17 +217,19 @@ std::shared_ptr<DeviceTables> get_tables(int device, const char* path, int N, do
     dt->bytes = 2 * layers + kSlabBytes;
     HIP_OK(hipMalloc(&dt->base, dt->bytes));
     {
-        // Node-major (m, c) slabs -> 32-byte cells (see kSlabBytes).
+        // Node-major (m, c) slabs -> 32-byte cells (see kSlabBytes); node
+        // row 51 reads row 49 and node column 50 reads column 48.
         std::vector<float> dev(dt->bytes / sizeof(float));
         auto slab = [&](const float* nodes, float* out) {
+            auto node = [&](uint32_t r, uint32_t q) { return nodes + 2 * ((r == 51 ? 49 : r) * 50 + (q == 50 ? 48 : q)); };
             for (uint32_t r = 0; r < kPairRows; ++r)
                 for (uint32_t q = 0; q < kRowCells; ++q)
                     for (uint32_t h = 0; h < 2; ++h) {  // entries (r, q) and (r, q + 1)
                         float* e = out + ((size_t)r * kRowBytes + q * 32 + h * 16) / sizeof(float);
-                        e[0] = nodes[2 * (r * 50 + q + h)];
-                        e[1] = nodes[2 * (r * 50 + q + h) + 1];
-                        e[2] = nodes[2 * ((r + 1) * 50 + q + h)];
-                        e[3] = nodes[2 * ((r + 1) * 50 + q + h) + 1];
+                        e[0] = node(r, q + h)[0];
+                        e[1] = node(r, q + h)[1];
+                        e[2] = node(r + 1, q + h)[0];
+                        e[3] = node(r + 1, q + h)[1];
                     }
         };
         const size_t slab_f = kSlabBytes / sizeof(float), node_f = (size_t)kGridNodes * 2;
