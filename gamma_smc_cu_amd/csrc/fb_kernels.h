@@ -1273,10 +1273,12 @@ __global__ void prior_table_kernel(DevTables t, float2* out, int64_t n)
 // fast 10^x keep that chain short.
 constexpr uint32_t kNoAdvance = 0xFFFFFFFFu;
 
-// FP16 slab layout: entry (r, q) (r < 50, q < 50) holds the four corners
+// FP16 slab layout: entry (r, q) (r <= 50, q <= 49) holds the four corners
 // (r, q), (r + 1, q), (r, q + 1), (r + 1, q + 1) as FP16 (m, c) pairs, 16
-// bytes, so a lookup is one aligned gather. 40,000 bytes per layer.
-constexpr uint32_t kQuadSlabEntries = 50 * 50;
+// bytes at 16 (50 r + q), so a lookup is one aligned gather; node row 51
+// repeats row 49 and node column 50 repeats column 48 (the edge cells of
+// the FP32 tables, so the index needs no cap). 40,800 bytes per layer.
+constexpr uint32_t kQuadSlabEntries = 51 * 50;
 
 struct FwdArgs {
     const uint64_t* packed;
@@ -1284,7 +1286,7 @@ struct FwdArgs {
     int64_t W;
     int64_t S;
     const uint32_t* desc;   // [S + 64] forward-only gap descriptors
-    const uint4* h2q;       // [N][50 * 50] quad entries
+    const uint4* h2q;       // [N][51 * 50] quad entries
     int32_t N;
     const int32_t* pair_i;
     const int32_t* pair_j;
@@ -1301,17 +1303,23 @@ __device__ __forceinline__ float2 half2_bits(uint32_t v)
 }
 
 // Bilinear lookup on one FP16 slab, the arithmetic of the FP32 lookup
-// (gamma_smc_flow.cu:1139-1172).
+// (gamma_smc_flow.cu:1139-1172). The walk's time is the latency of its
+// chain of dependent instructions per site, so the entry's byte offset is
+// one multiply-add from the mean index: 16 (50 lm + lc) + 2^23, an exact
+// integer in float whose bits are the offset plus 0x4B000000 (taken off the
+// layer base instead of converting), with no cap on the index (edge cells).
+constexpr uint32_t kFloatIntBias = 0x4B000000u;  // bits of 2^23
 __device__ __forceinline__ void lookup_h2(const uint4* __restrict__ layer, float& m, float& c)
 {
     float fm = (m - -5.0f) * kMeanStepInv;
     float fc = (c - -2.0f) * kCvStepInv;
     fm = __builtin_amdgcn_fmed3f(fm, 0.0f, 50.0f);
     fc = __builtin_amdgcn_fmed3f(fc, 0.0f, 49.0f);
-    const float lm = fminf(floorf(fm), 49.0f), lc = fminf(floorf(fc), 48.0f);
+    const float lm = floorf(fm), lc = floorf(fc);
     const float wm = fm - lm, wc = fc - lc;
-    // 32-bit byte offset from the layer base (scalar base + vector offset)
-    const uint4 e = *global_at<const uint4>((void*)layer, (uint32_t)fmaf(lm, 50.0f, lc) << 4);
+    // scalar layer base (less the bias) + 32-bit vector offset
+    const uint32_t off = __float_as_uint(fmaf(lm, 800.0f, fmaf(lc, 16.0f, 8388608.0f)));
+    const uint4 e = *global_at<const uint4>((void*)((uintptr_t)layer - kFloatIntBias), off);
     const float2 v00 = half2_bits(e.x), v10 = half2_bits(e.y), v01 = half2_bits(e.z), v11 = half2_bits(e.w);
     const float omwm = 1.0f - wm, omwc = 1.0f - wc;
     m = fmaf(wm, fmaf(wc, v11.x, omwc * v10.x), omwm * fmaf(wc, v01.x, omwc * v00.x));
@@ -1368,9 +1376,9 @@ __global__ __launch_bounds__(64) void fwd_only_kernel(FwdArgs a)
                 // halving folded into the constant (both exact rescalings)
                 const float alpha = __builtin_amdgcn_exp2f(c * -6.64385618977472470f) + 1.0f;
                 const float l2 = __builtin_amdgcn_logf(alpha);
-                const float a_log = l2 * 0.30102999566398120f;
                 const float b_log = fmaf(-2.0f, c, -m);
-                m = het ? a_log - b_log : m;
+                // log10(alpha) - b_log in one fused step
+                m = het ? fmaf(l2, 0.30102999566398120f, -b_log) : m;
                 c = het ? l2 * -0.15051499783199060f : c;
             }
             // 10^m by the hardware 2^x: table states lie in the grid box
@@ -1440,14 +1448,15 @@ __global__ void to_half2_quads_kernel(const float2* __restrict__ src, uint4* __r
     if (i >= n_layers * (int64_t)kQuadSlabEntries) return;
     const int64_t layer = i / kQuadSlabEntries;
     const int e = (int)(i - layer * kQuadSlabEntries);
-    const int r = e / 50, q = e % 50, q1 = q < 49 ? q + 1 : q;
+    const int r = e / 50, q = e % 50;
+    const int r1 = r < 50 ? r + 1 : 49, q1 = q < 49 ? q + 1 : 48;
     const float2* nodes = src + layer * kGridNodes;
     auto bits = [](float2 v) {
         const __half2 h = __floats2half2_rn(v.x, v.y);
         return *reinterpret_cast<const uint32_t*>(&h);
     };
-    dst[i] = make_uint4(bits(nodes[r * 50 + q]), bits(nodes[(r + 1) * 50 + q]), bits(nodes[r * 50 + q1]),
-                        bits(nodes[(r + 1) * 50 + q1]));
+    dst[i] = make_uint4(bits(nodes[r * 50 + q]), bits(nodes[r1 * 50 + q]), bits(nodes[r * 50 + q1]),
+                        bits(nodes[r1 * 50 + q1]));
 }
 
 __global__ void reset_counters_kernel(int* counters)

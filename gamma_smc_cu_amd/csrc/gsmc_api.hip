@@ -156,7 +156,7 @@ struct DeviceTables {
     char* base = nullptr;  // fwd layers, bwd layers, stretch slab
     size_t bytes = 0;
     DevTables view{};
-    uint4* h2q = nullptr;   // forward-only path: cache[k] as FP16 quad entries, [N][50 * 50]
+    uint4* h2q = nullptr;   // forward-only path: cache[k] as FP16 quad entries, [N][51 * 50]
     ~DeviceTables()
     {
         if (base || h2q) (void)hipSetDevice(device);
