@@ -79,6 +79,8 @@ template <class T> __device__ __forceinline__ T* global_at(void* base, uint32_t 
     return (T*)((global_char*)base + byte_off);
 }
 
+// Table bases as lookup() takes them: each the table's address less
+// kFloatIntBias (see lookup()), never dereferenced without a biased offset.
 struct DevTables {
     const char* fwd;      // [N] layers of kLayerBytes
     const char* bwd;      // [N] layers of kLayerBytes
@@ -163,9 +165,19 @@ constexpr int kQuad0101 = 0x44;  // lanes read lanes 0, 1, 0, 1 of their quad
 constexpr int kQuad2323 = 0xEE;  // 2, 3, 2, 3
 constexpr int kQuad2301 = 0x4E;  // 2, 3, 0, 1
 
-// Bilinear lookup of one table slab at (m, c); updates (m, c). slab (0 or
-// kSlabBytes) selects the hom or het slab of the layer. Every lane of the
-// wave must call it together.
+// Byte offsets into the tables are formed by exact float multiply-adds
+// from the cell indices; a float in [2^23, 2^24) holding an integer v has
+// the bits kFloatIntBias + v, so the offset is read from those bits with
+// the bias taken off the (scalar) table base instead of converting.
+constexpr uint32_t kFloatIntBias = 0x4B000000u;  // bits of 2^23
+constexpr uint32_t kHomSlab = kFloatIntBias;     // biased offset of a layer's hom slab
+constexpr uint32_t kHetSlab = kFloatIntBias | kSlabBytes;
+
+// Bilinear lookup of one table slab at (m, c); updates (m, c). layer: the
+// layer's address less kFloatIntBias (DevTables); slab (kHomSlab or
+// kHetSlab: the slab's byte offset, 0 or kSlabBytes, as biased float bits)
+// selects the hom or het slab of the layer. Every lane of the wave must call
+// it together.
 //
 // Same values as gamma_smc_flow.cu:408-430 in fewer instructions: the clamp
 // is one med3 (identical for any non-NaN index) and the cell corner is
@@ -196,18 +208,22 @@ __device__ __forceinline__ void lookup(const char* __restrict__ layer, uint32_t 
     const float lc = floorf(fc);
     const float wm = fm - lm;
     const float wc = fc - lc;
-    // cell (lm, lc) at 32-byte unit 50 lm + lc, an exact small integer in float
-    const uint32_t off = ((uint32_t)fmaf(lm, (float)kRowCells, lc) << 5) + slab;
+    // cell (lm, lc) at byte 32 (50 lm + lc) of the slab: biased float bits
+    const uint32_t off = __float_as_uint(fmaf(lm, (float)kRowBytes, fmaf(lc, 32.0f, __uint_as_float(slab))));
     const bool upper = (__lane_id() & 2u) != 0;
     // first gather: lanes 0, 1 their entry (r, q + 1); lanes 2, 3 entry (r, q) of lanes 0, 1
     const uint32_t a1 = quad_perm<kQuad0101>(off) + (upper ? 0u : 16u);
     // second gather: lanes 2, 3 their entry (r, q + 1); lanes 0, 1 entry (r, q) of lanes 2, 3
     const uint32_t a2 = quad_perm<kQuad2323>(off) + (upper ? 16u : 0u);
-    const float4 x = *reinterpret_cast<const float4*>(layer + a1);
-    const float4 y = *reinterpret_cast<const float4*>(layer + a2);
+    // scalar base (the table bases carry the bias) + 32-bit vector offsets
+    const global_char* base = (const global_char*)layer;
+    typedef float v4f __attribute__((ext_vector_type(4)));
+    typedef __attribute__((address_space(1))) const v4f global_v4f;
+    const v4f x = *reinterpret_cast<global_v4f*>(base + a1);
+    const v4f y = *reinterpret_cast<global_v4f*>(base + a2);
     // own entry (r, q + 1) = {v01, v11}; the partner's copy of own entry (r, q) = {v00, v10}
-    const float4 e1 = upper ? y : x;
-    const float4 z = upper ? x : y;
+    const v4f e1 = upper ? y : x;
+    const v4f z = upper ? x : y;
     const float omwm = 1.0f - wm, omwc = 1.0f - wc;
     const float p00m = omwc * quad_perm<kQuad2301>(z.x), p10m = omwc * quad_perm<kQuad2301>(z.z);
     const float p00c = omwc * quad_perm<kQuad2301>(z.y), p10c = omwc * quad_perm<kQuad2301>(z.w);
@@ -224,7 +240,7 @@ __device__ __forceinline__ const char* layer_of(const char* base, uint32_t d)
 // (remaining bp, observed site).
 __device__ __forceinline__ void fwd_segment(const DevTables& t, uint32_t d, uint32_t slab, float& m, float& c)
 {
-    for (uint32_t q = d >> kDescLayerBits; q != 0; --q) lookup(t.stretch, 0, m, c);
+    for (uint32_t q = d >> kDescLayerBits; q != 0; --q) lookup(t.stretch, kHomSlab, m, c);
     lookup(layer_of(t.fwd, d), slab, m, c);
 }
 
@@ -233,7 +249,7 @@ __device__ __forceinline__ void fwd_segment(const DevTables& t, uint32_t d, uint
 __device__ __forceinline__ void bwd_segment(const DevTables& t, uint32_t d, uint32_t slab, float& m, float& c)
 {
     lookup(layer_of(t.bwd, d), slab, m, c);
-    for (uint32_t q = d >> kDescLayerBits; q != 0; --q) lookup(t.stretch, 0, m, c);
+    for (uint32_t q = d >> kDescLayerBits; q != 0; --q) lookup(t.stretch, kHomSlab, m, c);
 }
 
 // Schraudolph-style 10^x as upstream gamma_smc reconstructs it
@@ -264,16 +280,16 @@ __device__ __forceinline__ uint32_t het_bit(uint64_t bits, int k)
     return (word >> (k & 31)) & 1u;
 }
 
-// Byte offset of the slab site k of a het mask selects: 0 (hom) or
-// kSlabBytes (het): a sign-extending bit extract (0 or all ones) and a mask,
-// on 32 bits. The extract is written out because the compiler otherwise
-// rewrites the pair into a 64-bit compare and a select (four instructions).
+// The slab site k of a het mask selects, as lookup() takes it: kHomSlab or
+// kHetSlab: a bit extract (0 or 1) and one 24-bit multiply-add, on 32
+// bits. The extract is written out because the compiler otherwise rewrites
+// it into a 64-bit compare and a select (four instructions).
 __device__ __forceinline__ uint32_t het_slab(uint64_t bits, int k)
 {
     const uint32_t word = k < 32 ? (uint32_t)bits : (uint32_t)(bits >> 32);
     uint32_t ext;
-    asm("v_bfe_i32 %0, %1, %2, 1" : "=v"(ext) : "v"(word), "n"(k & 31));
-    return ext & kSlabBytes;
+    asm("v_bfe_u32 %0, %1, %2, 1" : "=v"(ext) : "v"(word), "n"(k & 31));
+    return __umul24(ext, kSlabBytes) + kFloatIntBias;
 }
 
 // K consecutive site descriptors from s (the array is padded by 64 entries
@@ -1129,7 +1145,7 @@ __global__ __launch_bounds__(64) void lookup_probe_kernel(DevTables t, int32_t n
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
             const uint32_t hs = probe_hash(gid * 13u + (uint32_t)r * 7u + (uint32_t)k);
-            lookup(layer, (hs & 1023u) < 159u ? kSlabBytes : 0u, m[k], c[k]);
+            lookup(layer, (hs & 1023u) < 159u ? kHetSlab : kHomSlab, m[k], c[k]);
         }
     }
     out[gid] = m[0] + c[0] + m[1] + c[1];
@@ -1256,7 +1272,7 @@ __global__ void prior_table_kernel(DevTables t, float2* out, int64_t n)
     float m = 0.0f, c = 0.0f;
     if (writer) out[0] = make_float2(m, c);
     for (int64_t q = 1; q < n; ++q) {
-        lookup(t.stretch, 0u, m, c);
+        lookup(t.stretch, kHomSlab, m, c);
         if (writer) out[q] = make_float2(m, c);
     }
 }
@@ -1308,7 +1324,6 @@ __device__ __forceinline__ float2 half2_bits(uint32_t v)
 // one multiply-add from the mean index: 16 (50 lm + lc) + 2^23, an exact
 // integer in float whose bits are the offset plus 0x4B000000 (taken off the
 // layer base instead of converting), with no cap on the index (edge cells).
-constexpr uint32_t kFloatIntBias = 0x4B000000u;  // bits of 2^23
 __device__ __forceinline__ void lookup_h2(const uint4* __restrict__ layer, float& m, float& c)
 {
     float fm = (m - -5.0f) * kMeanStepInv;

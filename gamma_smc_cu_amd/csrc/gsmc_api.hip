@@ -240,9 +240,11 @@ std::shared_ptr<DeviceTables> get_tables(int device, const char* path, int N, do
         slab(host.stretch_last.data(), dev.data() + 2 * layers / sizeof(float));
         HIP_OK(hipMemcpy(dt->base, dev.data(), dt->bytes, hipMemcpyHostToDevice));
     }
-    dt->view.fwd = dt->base;
-    dt->view.bwd = dt->base + layers;
-    dt->view.stretch = dt->base + 2 * layers;
+    // the kernels add offsets biased by kFloatIntBias (fb_kernels.h::lookup)
+    const uintptr_t b0 = reinterpret_cast<uintptr_t>(dt->base) - kFloatIntBias;
+    dt->view.fwd = reinterpret_cast<const char*>(b0);
+    dt->view.bwd = reinterpret_cast<const char*>(b0 + layers);
+    dt->view.stretch = reinterpret_cast<const char*>(b0 + 2 * layers);
     {
         // FP16 copy of the composed transitions, rounded on the device as
         // the reference rounds them on the host (__floats2half2_rn), laid
