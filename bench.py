@@ -58,11 +58,11 @@ REFERENCE_E2E_S = 23.659  # benchmarks/pairwise_scaling/results.csv:22 (1x B200,
 # counter summary under profiles/ gives the commands. Any other kernel
 # source reports traffic null.
 MEASURED_TRAFFIC = {
-    "kernel_sha256": "ce0011d09d4fc04a7c15c7961770907f5bc119754767235ac2454a83264aed95",
+    "kernel_sha256": "a1ae1fc2ce959aa289ac6640cd319b569be20e449d9900c3da306b384ef10762",
     "entries": {
-        "chromosome/K8/np1/dynamic/xcd0": 177.77e9,
-        "chromosome/K8/np1/persistent/xcd0": 177.90e9,
-        "chromosome/K8/np1/static/xcd0": 180.74e9,
+        "chromosome/K8/np1/dynamic/xcd0": 190.33e9,
+        "chromosome/K8/np1/persistent/xcd0": 186.56e9,
+        "chromosome/K8/np1/static/xcd0": 197.63e9,
     },
 }
 
