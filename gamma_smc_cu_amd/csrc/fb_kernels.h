@@ -1453,6 +1453,24 @@ __global__ __launch_bounds__(64) void fwd_only_kernel(FwdArgs a)
     }
 }
 
+// Node-major slabs ([n_slabs][2550] (m, c) nodes, mean-major) to the
+// 32-byte cells the lookups read (kSlabBytes per slab): entry e = 2 (50 r +
+// q) + h of a slab holds nodes (r, q + h) and (r + 1, q + h), node row 51
+// reading row 49 and node column 50 reading column 48 (the edge cells).
+__global__ void cells_from_nodes_kernel(const float2* __restrict__ nodes, float4* __restrict__ cells, int64_t n_slabs)
+{
+    constexpr int kEntries = (int)(kSlabBytes / 16);  // 5,100
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_slabs * kEntries) return;
+    const int64_t s = i / kEntries;
+    const int e = (int)(i - s * kEntries);
+    const int r = e / 100, col = (e % 100) / 2 + (e & 1);
+    const int q = col == 50 ? 48 : col, r1 = r == 50 ? 49 : r + 1;
+    const float2* slab = nodes + s * kGridNodes;
+    const float2 lo = slab[r * 50 + q], hi = slab[r1 * 50 + q];
+    cells[i] = make_float4(lo.x, lo.y, hi.x, hi.y);
+}
+
 // FP32 (m, c) nodes [N][51][50] to FP16 quad entries [N][50][50] (round to
 // nearest even, as the reference rounds its table with __floats2half2_rn):
 // entry (r, q) = nodes (r, q), (r + 1, q), (r, q + 1), (r + 1, q + 1); the

@@ -167,6 +167,14 @@ struct DeviceTables {
 
 std::mutex g_tab_mu;
 std::map<std::string, std::weak_ptr<DeviceTables>> g_tabs;
+// The most recently used table set of each device stays alive after its
+// last context closes, as the reference's process-wide table cache does
+// (bindings.cpp:2016-2050, rebuilt only when the parameters change; there
+// one set per process, here one per device), so back-to-back contexts and
+// one-shot calls on the same parameters skip the build. Never destroyed:
+// the process's end frees the device memory (a static destructor would run
+// after the HIP runtime's).
+auto* g_tabs_last = new std::map<int, std::shared_ptr<DeviceTables>>();
 
 std::string table_key(int device, const FlowField& ff, int N, float rho, float mu, double rho64, double mu64,
                       int precision)
@@ -206,7 +214,10 @@ std::shared_ptr<DeviceTables> get_tables(int device, const char* path, int N, do
     std::lock_guard<std::mutex> lk(g_tab_mu);
     auto it = g_tabs.find(key);
     if (it != g_tabs.end()) {
-        if (auto sp = it->second.lock()) return sp;
+        if (auto sp = it->second.lock()) {
+            (*g_tabs_last)[device] = sp;
+            return sp;
+        }
     }
     HostTables host;
     build_tables(ff, N, srho, smu, srho64, smu64, precision, host);
@@ -216,29 +227,21 @@ std::shared_ptr<DeviceTables> get_tables(int device, const char* path, int N, do
     const size_t layers = (size_t)N * kLayerBytes;
     dt->bytes = 2 * layers + kSlabBytes;
     HIP_OK(hipMalloc(&dt->base, dt->bytes));
+    float2* node_tmp = nullptr;
     {
-        // Node-major (m, c) slabs -> 32-byte cells (see kSlabBytes); node
-        // row 51 reads row 49 and node column 50 reads column 48.
-        std::vector<float> dev(dt->bytes / sizeof(float));
-        auto slab = [&](const float* nodes, float* out) {
-            auto node = [&](uint32_t r, uint32_t q) { return nodes + 2 * ((r == 51 ? 49 : r) * 50 + (q == 50 ? 48 : q)); };
-            for (uint32_t r = 0; r < kPairRows; ++r)
-                for (uint32_t q = 0; q < kRowCells; ++q)
-                    for (uint32_t h = 0; h < 2; ++h) {  // entries (r, q) and (r, q + 1)
-                        float* e = out + ((size_t)r * kRowBytes + q * 32 + h * 16) / sizeof(float);
-                        e[0] = node(r, q + h)[0];
-                        e[1] = node(r, q + h)[1];
-                        e[2] = node(r + 1, q + h)[0];
-                        e[3] = node(r + 1, q + h)[1];
-                    }
-        };
-        const size_t slab_f = kSlabBytes / sizeof(float), node_f = (size_t)kGridNodes * 2;
-        for (size_t k = 0; k < (size_t)N * 2; ++k) {
-            slab(host.fwd.data() + k * node_f, dev.data() + k * slab_f);
-            slab(host.bwd.data() + k * node_f, dev.data() + layers / sizeof(float) + k * slab_f);
-        }
-        slab(host.stretch_last.data(), dev.data() + 2 * layers / sizeof(float));
-        HIP_OK(hipMemcpy(dt->base, dev.data(), dt->bytes, hipMemcpyHostToDevice));
+        // Node-major (m, c) slabs (fwd, bwd, stretch: the cell slabs' order)
+        // go up as built and are expanded into 32-byte cells on the device.
+        const size_t n_slabs = (size_t)N * 4 + 1, node_slab = (size_t)kGridNodes * sizeof(float2);
+        HIP_OK(hipMalloc(&node_tmp, n_slabs * node_slab));
+        char* t = reinterpret_cast<char*>(node_tmp);
+        HIP_OK(hipMemcpy(t, host.fwd.data(), (size_t)N * 2 * node_slab, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(t + (size_t)N * 2 * node_slab, host.bwd.data(), (size_t)N * 2 * node_slab,
+                         hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(t + (size_t)N * 4 * node_slab, host.stretch_last.data(), node_slab, hipMemcpyHostToDevice));
+        const int64_t entries = (int64_t)n_slabs * (int64_t)(kSlabBytes / 16);
+        hipLaunchKernelGGL(cells_from_nodes_kernel, dim3((unsigned)((entries + 255) / 256)), dim3(256), 0, nullptr,
+                           node_tmp, reinterpret_cast<float4*>(dt->base), (int64_t)n_slabs);
+        HIP_OK(hipGetLastError());
     }
     // the kernels add offsets biased by kFloatIntBias (fb_kernels.h::lookup)
     const uintptr_t b0 = reinterpret_cast<uintptr_t>(dt->base) - kFloatIntBias;
@@ -260,8 +263,10 @@ std::shared_ptr<DeviceTables> get_tables(int device, const char* path, int N, do
         HIP_OK(hipGetLastError());
         HIP_OK(hipDeviceSynchronize());
         HIP_OK(hipFree(tmp));
+        HIP_OK(hipFree(node_tmp));
     }
     g_tabs[key] = dt;
+    (*g_tabs_last)[device] = dt;
     for (auto i = g_tabs.begin(); i != g_tabs.end();) i = i->second.expired() ? g_tabs.erase(i) : std::next(i);
     return dt;
 }
@@ -1099,9 +1104,19 @@ void run_to_host(gsmc_context* ctx, const int32_t* pi, const int32_t* pj, int64_
         if (slot_max) ctx->slots[(size_t)s]->out.ensure(slot_max, ctx->device);
     }
 
+    // A task holding every pair writes whole host rows: its planes go
+    // straight into the caller's arrays when small (no staging buffers, no
+    // host scatter: what small one-shot decodes spend most of their host
+    // time on); larger ones stream through pinned staging, which the
+    // runtime's copies from pageable memory do not match in bandwidth.
+    constexpr size_t kDirectCopyMax = (size_t)64 << 20;
+    auto direct = [&](const Task& t) { return t.np == P && (size_t)t.rows * t.np * 4 <= kDirectCopyMax; };
+    bool need_stage = false;
+    for (const Task& t : tasks) need_stage = need_stage || !direct(t);
     std::vector<char> slot_used((size_t)n_slots, 0);
     const size_t stage_bytes = (size_t)256 << 20;
-    for (int s = 0; s < 2; ++s) ctx->stage[s].ensure(stage_bytes);
+    if (need_stage)
+        for (int s = 0; s < 2; ++s) ctx->stage[s].ensure(stage_bytes);
     int stage_next = 0;
     bool stage_busy[2] = {false, false};
     struct Pending {
@@ -1124,6 +1139,13 @@ void run_to_host(gsmc_context* ctx, const int32_t* pi, const int32_t* pj, int64_
         const size_t plane = (size_t)t.rows * t.np;
         const int64_t rows_per_piece = std::max<int64_t>(1, (int64_t)(stage_bytes / 4 / std::max<int64_t>(t.np, 1)));
         HIP_OK(hipStreamWaitEvent(ctx->copy, sl.ev_kernel, 0));
+        if (direct(t)) {
+            for (int o = 0; o < nout; ++o)
+                HIP_OK(hipMemcpyAsync(host_out[o] + (size_t)t.row0 * P, sl.out.as<float>() + (size_t)o * plane,
+                                      plane * 4, hipMemcpyDeviceToHost, ctx->copy));
+            HIP_OK(hipEventRecord(sl.ev_free, ctx->copy));
+            return;
+        }
         for (int o = 0; o < nout; ++o) {
             const float* src = sl.out.as<float>() + (size_t)o * plane;
             for (int64_t r = 0; r < t.rows; r += rows_per_piece) {
