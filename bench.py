@@ -27,7 +27,9 @@ The one JSON line also carries:
   reference's published 23.659 s;
 * ``biobank``, ``ragged``, ``serving``: compact runs of the other workloads
   (``--workload`` runs any of them alone, at full length), each with its
-  own checks.
+  own checks;
+* ``plain``: the reference's small one-shot call (190 pairs x 17,339
+  sites, ``gamma_smc_flow_cached_fb``) beside its published 0.0333 s.
 """
 
 from __future__ import annotations
@@ -222,6 +224,47 @@ def cpu_baseline_and_checks(G, pos, pairs, P, S, core, flank, Ne, kmu, krho, alo
         checks["fp64_error_within_tolerance"] = bool(fp64["max"] <= 1e-2 and fp64["median"] <= 1e-3)
         log(f"fp64 error: max {fp64['max']:.2e} median {fp64['median']:.2e} ({time.perf_counter() - t1:.1f}s)")
     return checks, cpu, fp64
+
+
+REFERENCE_PLAIN_S = 0.0333  # benchmarks/test_suite_stdpopsim/results/config_000.json (B200 MIG slice)
+
+
+def plain_small():
+    """The reference's small-job figure (run_one.py:244-260): a one-shot
+    ``_core.gamma_smc_flow_cached_fb`` over all 190 pairs of 20 haplotypes,
+    mean only, auto theta, min of 3 after a warm-up. Its input there is a
+    5 Mb msprime simulation with 17,339 sites (config_000.json); here a 1/k
+    cohort of the same shape (seed 1). Checked bit for bit against the
+    oracle."""
+    from gamma_smc_cu_amd import _core, synthetic
+    from gamma_smc_cu_amd.flow_field import DEFAULT_PATH
+    from gamma_smc_cu_amd.infer import _estimate_scaled_params
+    from oracle import oracle as O
+
+    n, S, span, Ne = 20, 17_339, 5_000_000, 1e4
+    G, pos = synthetic.cohort(n, S, span, 1)
+    pairs = [(i, j) for i in range(n) for j in range(i + 1, n)]
+    kmu, krho = _estimate_scaled_params(G, pos, 1.29e-8, 1.2820402396300887e-8, Ne)
+    _core.gamma_smc_flow_cached_fb(G, pos, [(0, 1)], Ne, kmu, krho, DEFAULT_PATH, True, 0)  # warm-up
+    times, mean = [], None
+    for _ in range(3):
+        t0 = time.perf_counter()
+        out = _core.gamma_smc_flow_cached_fb(G, pos, pairs, Ne, kmu, krho, DEFAULT_PATH, True, 0)["mean"]
+        times.append(time.perf_counter() - t0)
+        mean = out if mean is None else mean
+    T = O.tables_f32(DEFAULT_PATH, 1000, Ne, kmu, krho)
+    ref = O.fb_window_f32(G, pos, pairs, T, Ne)["mean"]
+    t = min(times)
+    out = {"seconds": t, "all_seconds": times, "pair_sites_per_s": len(pairs) * S / t,
+           "reference_seconds": REFERENCE_PLAIN_S, "speedup_vs_reference": REFERENCE_PLAIN_S / t,
+           "call": "_core.gamma_smc_flow_cached_fb(G, pos, all 190 pairs, Ne, auto theta mu, rho, flow field, "
+                   "mean_only=True, 0), one-shot (context, decode, host output), min of 3 after a warm-up",
+           "input": f"{n} haplotypes x {S} sites over 5 Mb, 1/k spectrum, seed 1 (the reference: msprime, "
+                    "HomSap OutOfAfrica_3G09, 17,339 sites)",
+           "checks": {"oracle_bit_exact": bits_equal(mean, ref)}}
+    log(f"plain: gamma_smc_flow_cached_fb {1e3 * t:.1f} ms for {len(pairs)} pairs x {S} sites "
+        f"(reference {1e3 * REFERENCE_PLAIN_S:.1f} ms)")
+    return out
 
 
 def end_to_end(G, pos, sample_pairs, alone, rows):
@@ -795,6 +838,7 @@ def main():
         sub["biobank"] = b
         sub["ragged"] = run_ragged(args, WORKLOADS["ragged"], compact=True)
         sub["serving"] = run_serving(args, WORKLOADS["serving"], compact=True)
+        sub["plain"] = plain_small()
         for name, r in sub.items():
             checks.update({f"{name}.{k}": v for k, v in (r.get("checks") or {}).items()})
 
