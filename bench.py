@@ -29,7 +29,10 @@ The one JSON line also carries:
   (``--workload`` runs any of them alone, at full length), each with its
   own checks;
 * ``plain``: the reference's small one-shot call (190 pairs x 17,339
-  sites, ``gamma_smc_flow_cached_fb``) beside its published 0.0333 s.
+  sites, ``gamma_smc_flow_cached_fb``) beside its published 0.0333 s;
+* ``scaling``: ``infer()`` over all pairs at the reference's pairwise
+  scaling shapes (4,950 / 19,900 / 79,800 pairs) beside its published
+  times.
 """
 
 from __future__ import annotations
@@ -265,6 +268,54 @@ def plain_small():
     log(f"plain: gamma_smc_flow_cached_fb {1e3 * t:.1f} ms for {len(pairs)} pairs x {S} sites "
         f"(reference {1e3 * REFERENCE_PLAIN_S:.1f} ms)")
     return out
+
+
+# benchmarks/pairwise_scaling/scaling_results.csv:2-4 (1x B200): haplotypes, sites, seconds
+REFERENCE_SCALING = [(100, 99_501, 0.806373119354248), (200, 117_259, 4.169840574264526),
+                     (400, 130_296, 457.0374381542206)]
+
+
+def scaling_infer():
+    """``infer()`` over every pair as the reference's pairwise scaling run
+    calls it (run_stdpopsim_scaling.py:58-76): a one-pair warm-up, then all
+    pairs, mean only, auto theta, output into host arrays; 1/k cohorts of the
+    published shapes over 51 Mb (seed 19) in place of its stdpopsim chr22
+    simulations. Three columns of each checked bit for bit against the
+    oracle."""
+    import gamma_smc_cu_amd
+    from gamma_smc_cu_amd import synthetic
+    from gamma_smc_cu_amd.flow_field import DEFAULT_PATH
+    from gamma_smc_cu_amd.infer import _estimate_scaled_params
+    from oracle import oracle as O
+
+    out = []
+    for n, S, ref_s in REFERENCE_SCALING:
+        G, pos = synthetic.cohort(n, S, 51_000_000, 19)
+        gamma_smc_cu_amd.infer(G, pos, pairs=[(1, 0)], mean_only=True, auto_estimate_theta=True)
+        t0 = time.perf_counter()
+        r = gamma_smc_cu_amd.infer(G, pos, mean_only=True, auto_estimate_theta=True)
+        t = time.perf_counter() - t0
+        all_pairs = [(i, j) for i in range(n) for j in range(i)]
+        P = len(all_pairs)
+        cols = [0, P // 2, P - 1]
+        kmu, krho = _estimate_scaled_params(G, pos, 1.25e-8, 1e-8, 1e4)
+        ref = O.fb_window_f32(G, pos, [all_pairs[k] for k in cols], O.tables_f32(DEFAULT_PATH, 1000, 1e4, kmu, krho),
+                              1e4)["mean"]
+        ok = bits_equal(np.ascontiguousarray(r["mean"][:, cols]), ref)
+        out.append({"n_haplotypes": n, "n_pairs": P, "n_sites": S, "seconds": t, "pair_sites_per_s": P * S / t,
+                     "output_gb": 4.0 * P * S / 1e9, "reference_seconds": ref_s,
+                     "speedup_vs_reference": ref_s / t, "checks": {"oracle_bit_exact_3_columns": ok}})
+        log(f"scaling: infer() {n} haplotypes ({P} pairs) x {S} sites: {t:.2f}s (reference {ref_s:.2f}s)")
+        del r, G, pos
+        gc.collect()
+    return {"call": "infer(G, pos, mu=1.25e-8, rho=1e-8, Ne=10000, mean_only=True, auto_estimate_theta=True), "
+                    "all pairs, after a one-pair warm-up; (S, P) float32 host output",
+            "input": "1/k-spectrum cohorts of the published shapes over 51 Mb, seed 19 (the reference: stdpopsim "
+                     "chr22 simulations)",
+            "reference_hardware": "1x NVIDIA B200 (benchmarks/pairwise_scaling/scaling_results.csv:2-4)",
+            "runs": out,
+            "checks": {f"n{r['n_haplotypes']}_oracle_bit_exact": r["checks"]["oracle_bit_exact_3_columns"]
+                       for r in out}}
 
 
 def end_to_end(G, pos, sample_pairs, alone, rows):
@@ -839,6 +890,7 @@ def main():
         sub["ragged"] = run_ragged(args, WORKLOADS["ragged"], compact=True)
         sub["serving"] = run_serving(args, WORKLOADS["serving"], compact=True)
         sub["plain"] = plain_small()
+        sub["scaling"] = scaling_infer()
         for name, r in sub.items():
             checks.update({f"{name}.{k}": v for k, v in (r.get("checks") or {}).items()})
 
