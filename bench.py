@@ -55,6 +55,8 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 REFERENCE_E2E_S = 23.659  # benchmarks/pairwise_scaling/results.csv:22 (1x B200, all 63,190 pairs)
+# results.csv:17-21: the same call on the first k shuffled pairs
+REFERENCE_E2E_SUBSETS = [(1, 1.1561), (10, 1.2289), (100, 1.3497), (1000, 1.6851), (10000, 4.6351)]
 
 # HBM bytes per decode launch pair measured with rocprofv3 FETCH_SIZE and
 # WRITE_SIZE passes (each scaled by what the counter reads for the
@@ -330,6 +332,13 @@ def end_to_end(G, pos, sample_pairs, alone, rows):
     np.random.default_rng(42).shuffle(all_pairs)
     gamma_smc_cu_amd.infer_blockwise(G, pos, mu=1.25e-8, rho=1e-8, Ne=10000, pairs=[(0, 1)], mean_only=True,
                                      auto_estimate_theta=True)
+    subsets = []
+    for k, ref_s in REFERENCE_E2E_SUBSETS:
+        t0 = time.perf_counter()
+        r = gamma_smc_cu_amd.infer_blockwise(G, pos, mu=1.25e-8, rho=1e-8, Ne=10000, pairs=all_pairs[:k],
+                                             mean_only=True, auto_estimate_theta=True, pair_batch_size=min(k, 30000))
+        subsets.append({"n_pairs": k, "seconds": time.perf_counter() - t0, "reference_seconds": ref_s})
+        del r
     t0 = time.perf_counter()
     r = gamma_smc_cu_amd.infer_blockwise(G, pos, mu=1.25e-8, rho=1e-8, Ne=10000, pairs=all_pairs, mean_only=True,
                                          auto_estimate_theta=True, pair_batch_size=min(len(all_pairs), 30000))
@@ -344,7 +353,10 @@ def end_to_end(G, pos, sample_pairs, alone, rows):
            "call": "infer_blockwise(G, pos, pairs=all pairs shuffled, mean_only=True, auto_estimate_theta=True, "
                    "pair_batch_size=30000), after a one-pair warm-up; G and positions in host memory, "
                    "output (S, P) float32 host array",
-           "checks": {"sampled_columns_equal_device_run": bits_equal(got, alone[np.asarray(rows)])}}
+           "checks": {"sampled_columns_equal_device_run": bits_equal(got, alone[np.asarray(rows)])},
+           "pair_subsets": subsets}
+    log("end to end, pair subsets: " + ", ".join(f"{x['n_pairs']}: {x['seconds']:.3f}s (reference {x['reference_seconds']}s)"
+                                                 for x in subsets))
     log(f"end to end: infer_blockwise {wall:.2f}s for {P} pairs x {S} sites "
         f"({out['output_gb']:.1f} GB to host; reference {REFERENCE_E2E_S}s)")
     del r
