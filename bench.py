@@ -828,7 +828,7 @@ def main():
     ap.add_argument("--workload", default="chromosome", choices=sorted(WORKLOADS))
     ap.add_argument("--tune", default="auto", choices=("auto", "none"),
                     help="auto: sweep launch shapes first and keep the fastest (results are identical)")
-    ap.add_argument("--checkpoint-k", type=int, default=16)
+    ap.add_argument("--checkpoint-k", type=int, default=8)
     ap.add_argument("--pairs-per-lane", type=int, default=1)
     ap.add_argument("--schedule", default="static", choices=("static", "persistent", "dynamic"))
     ap.add_argument("--xcd-order", type=int, default=0)

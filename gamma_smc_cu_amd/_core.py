@@ -352,7 +352,7 @@ class FlowContext:
         except Exception:
             pass
 
-    def set_tuning(self, checkpoint_k=16, pairs_per_lane=1, schedule="static", xcd_order=False,
+    def set_tuning(self, checkpoint_k=8, pairs_per_lane=1, schedule="static", xcd_order=False,
                    min_waves_per_launch=8192):
         """Launch shape (results are identical under every setting).
         ``schedule``: ``"static"`` (workgroup g decodes item g),

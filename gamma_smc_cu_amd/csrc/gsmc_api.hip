@@ -312,7 +312,7 @@ uint32_t descriptor(int steps, int N)
 using namespace gsmc;
 
 struct Tuning {
-    int K = 16;              // checkpoint interval = register window; 1: every forward state stored
+    int K = 8;               // checkpoint interval = recomputed window; 1: every forward state stored
     int np = 1;              // pairs per lane (mean-only kernel; 1 or 2)
     int schedule = 0;        // 0 static, 1 persistent queue
     int xcd_order = 0;       // static schedule: give each XCD a contiguous run of items

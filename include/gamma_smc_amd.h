@@ -131,7 +131,7 @@ void gsmc_context_destroy(gsmc_context* ctx);
 int gsmc_context_info(const gsmc_context* ctx, int64_t* n_haps, int64_t* n_sites, int* device_id, float* Ne);
 
 /* Decoder tuning; results do not depend on any of it. checkpoint_k: sites
- * per checkpoint window, 8 or 16 (default 16), or 1: the forward kernel
+ * per checkpoint window, 8 or 16 (default 8), or 1: the forward kernel
  * stores its state at every core site and the backward kernel reads it
  * back instead of recomputing it (mean, code and summary launches; CI /
  * posterior launches use 16). With the static schedule that takes 8 B per
