@@ -111,7 +111,7 @@ int gsmc_unpack(const uint64_t* packed, int64_t n_haps, int64_t n_sites, uint8_t
  * Contexts with equal tables share them; the tables a device used last
  * stay on it after their contexts are destroyed (about 370 MB at 1000
  * steps), as the reference's process-wide cache does, and are replaced
- * when a context needs a different set. */
+ * when a context needs a different set.
  * Replaces FlowContext.__init__ (src/bindings.cpp:2552-2670, binding
  * :3687-3699) and its table set-up ensure_cache (:2016-2157). */
 int gsmc_context_create(const uint8_t* G, int64_t n_haps, int64_t n_sites, const double* positions,
