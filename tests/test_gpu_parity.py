@@ -528,6 +528,11 @@ def test_log_codes_match_oracle(core, cohort_small, flow_field_path, bits):
         half_bin = got["log_max"] / (255.0 if bits == 8 else 15.0)
         inr = (ref >= 1.0) & (ref <= 2e5)
         assert np.all(np.abs(np.log(deq[inr]) - np.log(ref[inr])) <= half_bin * 0.5 + half_bin + 1e-6)
+        # 136 pairs (the list repeated): each pair's column is the same in
+        # whichever lane and 64-pair group decodes it
+        idx = np.arange(136) % len(pairs)
+        got_w = ctx.run_fb_codes([pairs[i] for i in idx], bits=bits, core_block_sites=csz, flank_sites=fl)["codes"]
+        assert np.array_equal(got_w, got["codes"][:, idx])
     if bits == 4:
         with pytest.raises(ValueError, match="even core_block_sites"):
             ctx.run_fb_codes(pairs, bits=4, core_block_sites=201, flank_sites=10)
