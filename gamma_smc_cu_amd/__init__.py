@@ -11,7 +11,8 @@ kernels through a C ABI (``include/gamma_smc_amd.h``).
 from . import _core
 from ._core import (FlowContext, bitpack, cuda_mem_info, gamma_smc_flow_cached_fb, gamma_smc_flow_cached_fwd,
                     unpack)
+from .genes import gene_stats
 from .infer import infer, infer_blockwise
 
 __all__ = ["FlowContext", "bitpack", "cuda_mem_info", "gamma_smc_flow_cached_fb", "gamma_smc_flow_cached_fwd",
-           "infer", "infer_blockwise", "unpack"]
+           "gene_stats", "infer", "infer_blockwise", "unpack"]

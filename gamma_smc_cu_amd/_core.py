@@ -46,7 +46,7 @@ EXPORTS = (
     "gsmc_batch_reserve", "gsmc_run_fb_summary", "gsmc_run_fwd", "gsmc_run_fwd_device", "gsmc_batch_capture",
     "gsmc_graph_launch", "gsmc_graph_destroy", "gsmc_context_set_state_budget", "gsmc_context_release_buffers",
     "gsmc_run_fwd_states", "gsmc_get_device", "gsmc_batch_buffer_bytes", "gsmc_run_fb_device_noalloc",
-    "gsmc_lookup_probe",
+    "gsmc_lookup_probe", "gsmc_run_fb_gene_stats",
 )
 
 
@@ -122,6 +122,8 @@ def _declare(L):
         "gsmc_batch_reserve": [_vp, ctypes.c_uint64, _i64, _i64],
         "gsmc_batch_buffer_bytes": [_vp, ctypes.POINTER(ctypes.c_uint64)],
         "gsmc_run_fb_summary": [_vp, _vp, _vp, _i64, _i64, _i64, _vp, _vp, _vp],
+        "gsmc_run_fb_gene_stats": [_vp, _vp, _vp, _i64, _i64, _i64, _i64, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp,
+                                   _vp, _vp, _vp],
         "gsmc_run_fwd": [_vp, _vp, _vp, _i64, _i32, _vp, _vp, _vp],
         "gsmc_run_fwd_device": [_vp, _vp, _vp, _i64, _i32, _vp, _vp, _vp, _vp],
         "gsmc_batch_capture": [_vp, _vp, _i64, _i32, ctypes.POINTER(_vp)],
@@ -450,6 +452,37 @@ class FlowContext:
                                             _ptr(out["site_mean"]), _ptr(out["site_min"]), _ptr(out["site_max"])))
         out["n_pairs"] = P
         return out
+
+    def run_fb_gene_stats(self, pairs, gene_sites, core_block_sites=0, flank_sites=0, bin_edges=None,
+                          accumulators=None):
+        """Per-gene statistics of the posterior means over ``pairs``, without
+        per-pair outputs: the accumulators of the reference's per-gene
+        pipeline (``analysis/genome_wide/infer_chromosome.py:157-227``) for
+        genes given as site ranges ``gene_sites[g] = (lo, hi)`` (sites
+        ``lo <= s < hi`` of this context). Returns ``count``, ``lin_sum``,
+        ``log_sum``, ``log_sq_sum``, ``min_lin``, ``min_log``, ``histogram``
+        and ``bin_edges``; pass the result back as ``accumulators`` to add
+        further pairs to it. ``core_block_sites <= 0`` decodes the full
+        sequence."""
+        from .genes import DEFAULT_BIN_EDGES, new_accumulators
+
+        gs = np.ascontiguousarray(np.asarray(gene_sites, dtype=np.int64).reshape(-1, 2))
+        edges = np.ascontiguousarray(DEFAULT_BIN_EDGES if bin_edges is None else bin_edges, dtype=np.float64)
+        if edges.ndim != 1 or edges.size < 2:
+            raise ValueError("bin_edges must hold at least two edges")
+        acc = new_accumulators(gs.shape[0], edges) if accumulators is None else accumulators
+        if acc["histogram"].shape != (gs.shape[0], edges.size - 1):
+            raise ValueError("accumulators do not match the genes and bins")
+        if flank_sites < 0:
+            raise ValueError("flank_sites must be non-negative.")
+        pi, pj = _split_pairs(pairs)
+        lo, hi = np.ascontiguousarray(gs[:, 0]), np.ascontiguousarray(gs[:, 1])
+        check(lib().gsmc_run_fb_gene_stats(self._h, _ptr(pi), _ptr(pj), pi.shape[0], int(core_block_sites),
+                                           int(flank_sites), gs.shape[0], _ptr(lo), _ptr(hi), edges.size - 1,
+                                           _ptr(edges), _ptr(acc["count"]), _ptr(acc["lin_sum"]),
+                                           _ptr(acc["log_sum"]), _ptr(acc["log_sq_sum"]), _ptr(acc["min_lin"]),
+                                           _ptr(acc["min_log"]), _ptr(acc["histogram"])))
+        return acc
 
     def checkpoint_bytes(self, n_sites, core_block_sites, flank_sites, n_pairs):
         out = ctypes.c_uint64()

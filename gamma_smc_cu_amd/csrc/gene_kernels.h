@@ -1,0 +1,168 @@
+// Device code of the per-gene statistics of posterior means (gfx950).
+//
+// The production caller of the decoder (analysis/genome_wide/
+// infer_chromosome.py:157-227) decodes pair chunks to host and reduces the
+// (sites, pairs) means there, per gene: for each pair the mean over the
+// gene's sites of max(mean, 1) and of its natural log, then over the pairs
+// the count, the sums of both and of the squared log, their minima and a
+// histogram of the per-pair log. Here the decoded rows stay on the device
+// and two kernels reduce them:
+//   gene_piece_sums_kernel  per pair and piece (a run of sites inside one
+//                           elementary segment of the genes' boundaries),
+//                           FP64 sums of max(mean, 1) and log2(max(mean, 1))
+//                           in site order;
+//   gene_reduce_kernel      per gene, the pieces it covers summed per pair in
+//                           piece order, then the pairs reduced in a fixed
+//                           tree, accumulated into the running totals of
+//                           earlier pair chunks.
+// Both are streaming kernels: the first reads each covered row once (4 B
+// per pair-site, HBM-bound), the second reads 16 B per pair and covered
+// piece.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gsmc {
+
+constexpr int kMaxGeneBins = 1024;  // histogram bins held in LDS per gene
+
+struct GenePieceArgs {
+    const float* mean;         // decoded rows [S][P] (row stride P)
+    int64_t P;
+    const int64_t* piece_lo;   // piece k covers rows [piece_lo[k], piece_hi[k])
+    const int64_t* piece_hi;
+    const int32_t* item_off;   // item i covers pieces [item_off[i], item_off[i + 1])
+    int64_t tiles;             // 64-pair tiles; block b is tile b % tiles of item b / tiles
+    double2* part;             // [n_pieces][P]: (sum of x, sum of log2 x), x = max(mean, 1)
+};
+
+// One wave per (64-pair tile, item). The row loop is unrolled by 8 so that
+// eight loads of the lane's column are in flight at once; the FP64 sums
+// take the sites in increasing order.
+__global__ __launch_bounds__(64) void gene_piece_sums_kernel(GenePieceArgs a)
+{
+    const int64_t b = blockIdx.x;
+    const int64_t tile = b % a.tiles, item = b / a.tiles;
+    const int64_t p = tile * 64 + threadIdx.x;
+    if (p >= a.P) return;
+    const float* col = a.mean + p;
+    const int32_t k0 = a.item_off[item], k1 = a.item_off[item + 1];
+    for (int32_t k = k0; k < k1; ++k) {
+        const int64_t lo = a.piece_lo[k], hi = a.piece_hi[k];
+        double sl = 0.0, sg = 0.0;
+        int64_t s = lo;
+        for (; s + 8 <= hi; s += 8) {
+            float x[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) x[u] = __builtin_nontemporal_load(col + (s + u) * a.P);
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const float v = fmaxf(x[u], 1.0f);
+                sl += (double)v;
+                sg += (double)__log2f(v);
+            }
+        }
+        for (; s < hi; ++s) {
+            const float v = fmaxf(__builtin_nontemporal_load(col + s * a.P), 1.0f);
+            sl += (double)v;
+            sg += (double)__log2f(v);
+        }
+        a.part[(size_t)k * (size_t)a.P + (size_t)p] = make_double2(sl, sg);
+    }
+}
+
+struct GeneReduceArgs {
+    const double2* part;       // [n_pieces][P] from gene_piece_sums_kernel
+    int64_t P;
+    const int32_t* gene_pb;    // gene g covers pieces [gene_pb[g], gene_pe[g])
+    const int32_t* gene_pe;
+    const int64_t* gene_n;     // sites of gene g
+    const double* edges;       // n_bins + 1 increasing histogram edges (natural log)
+    int n_bins;
+    int64_t* count;            // running totals, one per gene
+    double* lin_sum;
+    double* log_sum;
+    double* log_sq_sum;
+    double* min_lin;
+    double* min_log;
+    int64_t* hist;             // [n_genes][n_bins]
+};
+
+// One workgroup per gene. Thread t takes pairs t, t + 256, ...: the pair's
+// piece sums in piece order give its mean x and mean log x (log2 sum times
+// ln 2, over the gene's n sites); the thread keeps FP64 sums and minima,
+// and the bin of mean log x (the number of edges <= it, less one, clamped
+// to the bins: numpy.digitize and clip) is counted in LDS. A fixed tree
+// over the 256 threads joins the sums and minima, which thread 0 adds to
+// the running totals. Genes of fewer than 2 sites are left unchanged.
+__global__ __launch_bounds__(256) void gene_reduce_kernel(GeneReduceArgs a)
+{
+    constexpr double kLn2 = 0.69314718055994530942;
+    const int g = blockIdx.x;
+    const int t = threadIdx.x;
+    const int64_t n = a.gene_n[g];
+    if (n < 2) return;
+    __shared__ double s_edge[kMaxGeneBins + 1];
+    __shared__ uint32_t s_hist[kMaxGeneBins];
+    __shared__ double s_red[5][256];
+    for (int i = t; i <= a.n_bins; i += 256) s_edge[i] = a.edges[i];
+    for (int i = t; i < a.n_bins; i += 256) s_hist[i] = 0u;
+    __syncthreads();
+    const int32_t pb = a.gene_pb[g], pe = a.gene_pe[g];
+    const double inv_n = 1.0 / (double)n;
+    double sl = 0.0, sg = 0.0, sq = 0.0;
+    double ml = __builtin_inf(), mg = __builtin_inf();
+    for (int64_t p = t; p < a.P; p += 256) {
+        double xl = 0.0, xg = 0.0;
+        for (int32_t k = pb; k < pe; ++k) {
+            const double2 d = a.part[(size_t)k * (size_t)a.P + (size_t)p];
+            xl += d.x;
+            xg += d.y;
+        }
+        const double lin = xl * inv_n;
+        const double lg = (xg * kLn2) * inv_n;
+        sl += lin;
+        sg += lg;
+        sq += lg * lg;
+        ml = fmin(ml, lin);
+        mg = fmin(mg, lg);
+        int lo = 0, hi = a.n_bins + 1;  // count of edges <= lg, by bisection
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (s_edge[mid] <= lg)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        const int bin = min(max(lo - 1, 0), a.n_bins - 1);
+        atomicAdd(&s_hist[bin], 1u);
+    }
+    s_red[0][t] = sl;
+    s_red[1][t] = sg;
+    s_red[2][t] = sq;
+    s_red[3][t] = ml;
+    s_red[4][t] = mg;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            s_red[0][t] += s_red[0][t + w];
+            s_red[1][t] += s_red[1][t + w];
+            s_red[2][t] += s_red[2][t + w];
+            s_red[3][t] = fmin(s_red[3][t], s_red[3][t + w]);
+            s_red[4][t] = fmin(s_red[4][t], s_red[4][t + w]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        a.count[g] += a.P;
+        a.lin_sum[g] += s_red[0][0];
+        a.log_sum[g] += s_red[1][0];
+        a.log_sq_sum[g] += s_red[2][0];
+        a.min_lin[g] = fmin(a.min_lin[g], s_red[3][0]);
+        a.min_log[g] = fmin(a.min_log[g], s_red[4][0]);
+    }
+    for (int i = t; i < a.n_bins; i += 256) a.hist[(size_t)g * (size_t)a.n_bins + i] += (int64_t)s_hist[i];
+}
+
+}  // namespace gsmc

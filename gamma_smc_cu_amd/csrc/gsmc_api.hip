@@ -26,6 +26,7 @@
 
 #include "../../include/gamma_smc_amd.h"
 #include "fb_kernels.h"
+#include "gene_kernels.h"
 #include "tables.h"
 
 // Device scratch of gsmc_run_fb_device_noalloc: launch metadata, flank
@@ -347,6 +348,7 @@ struct gsmc_context {
     hipEvent_t ev_stage[2] = {nullptr, nullptr};   // staging buffer filled (copy -> host)
     std::unique_ptr<gsmc_batch> dev_batch;
     DevBuf sum_pairs, sum_site;  // per-site summary path: pair chunk, outputs and running values
+    DevBuf gene_pairs, gene_rows, gene_part, gene_meta, gene_acc;  // per-gene statistics path
     void* scratch = nullptr;     // this device's `scratch` (gsmc_run_fb_device_noalloc)
 
     ~gsmc_context();
@@ -1683,7 +1685,225 @@ int gsmc_run_fb_summary(gsmc_context* ctx, const int32_t* pi, const int32_t* pj,
 
 namespace gsmc {
 namespace {
-void launch_fwd_only(gsmc_context* ctx, const int32_t* d_pi, const int32_t* d_pj, int64_t P, int flags,
+
+// Work layout of a per-gene statistics call: the genes' boundaries cut the
+// sites into elementary segments; the segments some gene covers are split
+// into pieces of at most kGenePieceSites sites, and consecutive pieces are
+// grouped into items of at most kGeneItemSites sites (one wave per item
+// and 64-pair tile). A gene covers a contiguous run of pieces.
+struct GenePlan {
+    std::vector<int64_t> piece_lo, piece_hi;
+    std::vector<int32_t> item_off;
+    std::vector<int32_t> gene_pb, gene_pe;
+    std::vector<int64_t> gene_n;
+};
+
+constexpr int64_t kGenePieceSites = 2048;
+constexpr int64_t kGeneItemSites = 4096;
+
+static void plan_genes(int64_t S, int64_t n_genes, const int64_t* lo, const int64_t* hi, GenePlan& gp)
+{
+    std::vector<std::pair<int64_t, int>> ev;  // (site, +1 gene starts / -1 gene ends)
+    ev.reserve((size_t)n_genes * 2);
+    for (int64_t g = 0; g < n_genes; ++g) {
+        require(lo[g] >= 0 && lo[g] <= hi[g] && hi[g] <= S,
+                "gene " + std::to_string(g) + " site range [" + std::to_string(lo[g]) + ", " +
+                    std::to_string(hi[g]) + ") is not within [0, " + std::to_string(S) + "]");
+        if (hi[g] > lo[g]) {
+            ev.emplace_back(lo[g], 1);
+            ev.emplace_back(hi[g], -1);
+        }
+    }
+    std::sort(ev.begin(), ev.end());
+    int cover = 0;
+    for (size_t i = 0; i < ev.size();) {
+        const int64_t at = ev[i].first;
+        while (i < ev.size() && ev[i].first == at) cover += ev[i++].second;
+        if (cover > 0 && i < ev.size()) {
+            const int64_t end = ev[i].first;  // next boundary: the segment is [at, end)
+            for (int64_t s = at; s < end; s += kGenePieceSites) {
+                gp.piece_lo.push_back(s);
+                gp.piece_hi.push_back(std::min(end, s + kGenePieceSites));
+            }
+        }
+    }
+    require(gp.piece_lo.size() < (size_t)INT32_MAX, "too many gene pieces");
+    gp.item_off.push_back(0);
+    int64_t run = 0;
+    for (size_t k = 0; k < gp.piece_lo.size(); ++k) {
+        const int64_t len = gp.piece_hi[k] - gp.piece_lo[k];
+        if (run > 0 && run + len > kGeneItemSites) {
+            gp.item_off.push_back((int32_t)k);
+            run = 0;
+        }
+        run += len;
+    }
+    if (!gp.piece_lo.empty()) gp.item_off.push_back((int32_t)gp.piece_lo.size());
+    gp.gene_pb.resize((size_t)n_genes);
+    gp.gene_pe.resize((size_t)n_genes);
+    gp.gene_n.resize((size_t)n_genes);
+    for (int64_t g = 0; g < n_genes; ++g) {
+        gp.gene_pb[(size_t)g] = (int32_t)(std::lower_bound(gp.piece_lo.begin(), gp.piece_lo.end(), lo[g]) -
+                                          gp.piece_lo.begin());
+        gp.gene_pe[(size_t)g] = (int32_t)(std::lower_bound(gp.piece_lo.begin(), gp.piece_lo.end(), hi[g]) -
+                                          gp.piece_lo.begin());
+        gp.gene_n[(size_t)g] = hi[g] - lo[g];
+    }
+}
+}  // namespace
+}  // namespace gsmc
+
+int gsmc_run_fb_gene_stats(gsmc_context* ctx, const int32_t* pi, const int32_t* pj, int64_t P, int64_t core,
+                           int64_t flank, int64_t n_genes, const int64_t* gene_lo, const int64_t* gene_hi, int n_bins,
+                           const double* edges, int64_t* count, double* lin_sum, double* log_sum, double* log_sq_sum,
+                           double* min_lin, double* min_log, int64_t* histogram)
+{
+    return guarded([&] {
+        require(ctx != nullptr, "context is NULL");
+        require(P >= 0, "negative n_pairs");
+        require(n_genes >= 0, "negative n_genes");
+        require(flank >= 0, "flank_sites must be non-negative.");
+        require(n_bins >= 1 && n_bins <= kMaxGeneBins,
+                "n_bins must be in [1, " + std::to_string(kMaxGeneBins) + "]");
+        require(edges != nullptr, "edges is NULL");
+        for (int i = 0; i <= n_bins; ++i) require(std::isfinite(edges[i]), "histogram edges must be finite");
+        for (int i = 0; i < n_bins; ++i) require(edges[i] < edges[i + 1], "histogram edges must increase");
+        if (n_genes == 0) return;
+        require(gene_lo != nullptr && gene_hi != nullptr && count != nullptr && lin_sum != nullptr &&
+                    log_sum != nullptr && log_sq_sum != nullptr && min_lin != nullptr && min_log != nullptr &&
+                    histogram != nullptr,
+                "NULL argument");
+        const int64_t S = ctx->S;
+        GenePlan gp;
+        plan_genes(S, n_genes, gene_lo, gene_hi, gp);
+        if (P == 0 || S == 0 || gp.piece_lo.empty()) return;
+        require(pi != nullptr && pj != nullptr, "NULL argument");
+        require(n_genes <= INT32_MAX, "too many genes");
+        check_pairs(ctx, pi, pj, P);
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        HIP_OK(hipSetDevice(ctx->device));
+        if (!ctx->dev_batch) ctx->dev_batch = new_batch(ctx);
+        const int64_t c = core > 0 ? core : S;
+        const int64_t fl = core > 0 ? flank : 0;
+        const int64_t n_pieces = (int64_t)gp.piece_lo.size();
+        const int64_t n_items = (int64_t)gp.item_off.size() - 1;
+        // Pairs per chunk: decoded rows (4 B per site), piece sums (16 B per
+        // piece) and checkpoints (8 B every K sites) within 40 % of the
+        // memory the call can use (free memory plus the buffers it reuses).
+        size_t free_b = 0, total_b = 0;
+        HIP_OK(hipMemGetInfo(&free_b, &total_b));
+        const double avail = (double)free_b + (double)ctx->dev_batch->ck.bytes + (double)ctx->gene_rows.bytes +
+                             (double)ctx->gene_part.bytes;
+        const double per_pair = 4.0 * (double)S + 16.0 * (double)n_pieces +
+                                8.0 * (double)(S + 2 * fl) / ctx->tune.K + 64.0;
+        int64_t chunk = (int64_t)(0.4 * avail / per_pair) / 64 * 64;
+        chunk = std::max<int64_t>(chunk, 64);
+        chunk = std::min<int64_t>(chunk, (P + 63) / 64 * 64);
+        require((chunk / 64) * n_items <= INT32_MAX, "too many gene work items");
+        // device metadata: pieces, items, genes, edges; accumulators
+        const size_t o_hi = align256((size_t)n_pieces * 8);
+        const size_t o_item = o_hi + align256((size_t)n_pieces * 8);
+        const size_t o_pb = o_item + align256((size_t)(n_items + 1) * 4);
+        const size_t o_pe = o_pb + align256((size_t)n_genes * 4);
+        const size_t o_n = o_pe + align256((size_t)n_genes * 4);
+        const size_t o_edge = o_n + align256((size_t)n_genes * 8);
+        const size_t meta_bytes = o_edge + (size_t)(n_bins + 1) * 8;
+        const size_t a_lin = align256((size_t)n_genes * 8);
+        const size_t a_log = a_lin + align256((size_t)n_genes * 8);
+        const size_t a_sq = a_log + align256((size_t)n_genes * 8);
+        const size_t a_mlin = a_sq + align256((size_t)n_genes * 8);
+        const size_t a_mlog = a_mlin + align256((size_t)n_genes * 8);
+        const size_t a_hist = a_mlog + align256((size_t)n_genes * 8);
+        const size_t acc_bytes = a_hist + (size_t)n_genes * (size_t)n_bins * 8;
+        // the previous call's kernels may still read these buffers
+        HIP_OK(hipStreamSynchronize(ctx->compute));
+        ctx->gene_meta.ensure(meta_bytes, ctx->device, true);
+        ctx->gene_acc.ensure(acc_bytes, ctx->device, true);
+        ctx->gene_pairs.ensure((size_t)chunk * 2 * sizeof(int32_t), ctx->device, true);
+        ctx->gene_rows.ensure((size_t)chunk * (size_t)S * sizeof(float), ctx->device, true);
+        ctx->gene_part.ensure((size_t)chunk * (size_t)n_pieces * sizeof(double2), ctx->device, true);
+        char* meta = ctx->gene_meta.as<char>();
+        char* acc = ctx->gene_acc.as<char>();
+        auto h2d = [&](void* dst, const void* src, size_t bytes) {
+            if (bytes) HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->compute));
+        };
+        h2d(meta, gp.piece_lo.data(), (size_t)n_pieces * 8);
+        h2d(meta + o_hi, gp.piece_hi.data(), (size_t)n_pieces * 8);
+        h2d(meta + o_item, gp.item_off.data(), (size_t)(n_items + 1) * 4);
+        h2d(meta + o_pb, gp.gene_pb.data(), (size_t)n_genes * 4);
+        h2d(meta + o_pe, gp.gene_pe.data(), (size_t)n_genes * 4);
+        h2d(meta + o_n, gp.gene_n.data(), (size_t)n_genes * 8);
+        h2d(meta + o_edge, edges, (size_t)(n_bins + 1) * 8);
+        h2d(acc, count, (size_t)n_genes * 8);
+        h2d(acc + a_lin, lin_sum, (size_t)n_genes * 8);
+        h2d(acc + a_log, log_sum, (size_t)n_genes * 8);
+        h2d(acc + a_sq, log_sq_sum, (size_t)n_genes * 8);
+        h2d(acc + a_mlin, min_lin, (size_t)n_genes * 8);
+        h2d(acc + a_mlog, min_log, (size_t)n_genes * 8);
+        h2d(acc + a_hist, histogram, (size_t)n_genes * (size_t)n_bins * 8);
+
+        GenePieceArgs ga;
+        ga.piece_lo = reinterpret_cast<const int64_t*>(meta);
+        ga.piece_hi = reinterpret_cast<const int64_t*>(meta + o_hi);
+        ga.item_off = reinterpret_cast<const int32_t*>(meta + o_item);
+        ga.mean = ctx->gene_rows.as<float>();
+        ga.part = ctx->gene_part.as<double2>();
+        GeneReduceArgs gr;
+        gr.part = ga.part;
+        gr.gene_pb = reinterpret_cast<const int32_t*>(meta + o_pb);
+        gr.gene_pe = reinterpret_cast<const int32_t*>(meta + o_pe);
+        gr.gene_n = reinterpret_cast<const int64_t*>(meta + o_n);
+        gr.edges = reinterpret_cast<const double*>(meta + o_edge);
+        gr.n_bins = n_bins;
+        gr.count = reinterpret_cast<int64_t*>(acc);
+        gr.lin_sum = reinterpret_cast<double*>(acc + a_lin);
+        gr.log_sum = reinterpret_cast<double*>(acc + a_log);
+        gr.log_sq_sum = reinterpret_cast<double*>(acc + a_sq);
+        gr.min_lin = reinterpret_cast<double*>(acc + a_mlin);
+        gr.min_log = reinterpret_cast<double*>(acc + a_mlog);
+        gr.hist = reinterpret_cast<int64_t*>(acc + a_hist);
+
+        int32_t* d_pi = ctx->gene_pairs.as<int32_t>();
+        int32_t* d_pj = d_pi + chunk;
+        for (int64_t p0 = 0; p0 < P; p0 += chunk) {
+            const int64_t n = std::min(chunk, P - p0);
+            // the previous chunk's kernels read the pair buffer
+            HIP_OK(hipStreamSynchronize(ctx->compute));
+            h2d(d_pi, pi + p0, (size_t)n * sizeof(int32_t));
+            h2d(d_pj, pj + p0, (size_t)n * sizeof(int32_t));
+            JobSpec js;
+            js.blocks = make_blocks(S, c, fl);
+            js.pi = d_pi;
+            js.pj = d_pj;
+            js.P = n;
+            js.out.mean = ctx->gene_rows.as<float>();
+            launch_plan(ctx->dev_batch.get(), {js}, 0, ctx->compute);
+            ga.P = n;
+            ga.tiles = (n + 63) / 64;
+            hipLaunchKernelGGL(gene_piece_sums_kernel, dim3((unsigned)(ga.tiles * n_items)), dim3(64), 0,
+                               ctx->compute, ga);
+            HIP_OK(hipGetLastError());
+            gr.P = n;
+            hipLaunchKernelGGL(gene_reduce_kernel, dim3((unsigned)n_genes), dim3(256), 0, ctx->compute, gr);
+            HIP_OK(hipGetLastError());
+        }
+        auto d2h = [&](void* dst, const void* src, size_t bytes) {
+            HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->compute));
+        };
+        d2h(count, acc, (size_t)n_genes * 8);
+        d2h(lin_sum, acc + a_lin, (size_t)n_genes * 8);
+        d2h(log_sum, acc + a_log, (size_t)n_genes * 8);
+        d2h(log_sq_sum, acc + a_sq, (size_t)n_genes * 8);
+        d2h(min_lin, acc + a_mlin, (size_t)n_genes * 8);
+        d2h(min_log, acc + a_mlog, (size_t)n_genes * 8);
+        d2h(histogram, acc + a_hist, (size_t)n_genes * (size_t)n_bins * 8);
+        HIP_OK(hipStreamSynchronize(ctx->compute));
+    });
+}
+
+namespace gsmc {
+namespace {
+static void launch_fwd_only(gsmc_context* ctx, const int32_t* d_pi, const int32_t* d_pj, int64_t P, int flags,
                      float* d_mean, float* d_lower, float* d_upper, hipStream_t stream)
 {
     FwdArgs a;

@@ -211,6 +211,29 @@ int gsmc_run_fb_summary(gsmc_context* ctx, const int32_t* pair_i, const int32_t*
                         int64_t core_block_sites, int64_t flank_sites, float* site_mean, float* site_min,
                         float* site_max);
 
+/* Per-gene statistics of the posterior means of n_pairs pairs, without
+ * per-pair outputs. Gene g covers the context's sites [gene_lo[g],
+ * gene_hi[g]) (n_g sites); genes may overlap. With x = max(mean, 1) at each
+ * site, pair p's gene values are lin = (sum of x) / n_g and lg = (sum of
+ * ln x) / n_g (FP64 sums in site order of the FP32 means and their
+ * hardware log2, times ln 2). For each gene with n_g >= 2 the call adds
+ * n_pairs to count[g], the sums over pairs of lin, lg and lg^2 to lin_sum,
+ * log_sum and log_sq_sum, lowers min_lin / min_log to the pairs' minima,
+ * and counts each pair's lg in histogram[g][bin], bin = (number of edges
+ * <= lg) - 1 clamped to [0, n_bins - 1] (edges: n_bins + 1 increasing
+ * values, n_bins <= 1024). All outputs accumulate, so a caller covering
+ * the pairs in several calls initialises them once (0, 0, 0, 0, +inf,
+ * +inf, 0). Decodes blockwise as gsmc_run_fb_blockwise (core_block_sites
+ * <= 0: the full sequence) into device rows and reduces them there.
+ * Replaces the per-chunk host accumulation of the reference's per-gene
+ * pipeline (analysis/genome_wide/infer_chromosome.py:157-227), which reads
+ * each chunk's (sites, pairs) means back from infer_blockwise first. */
+int gsmc_run_fb_gene_stats(gsmc_context* ctx, const int32_t* pair_i, const int32_t* pair_j, int64_t n_pairs,
+                           int64_t core_block_sites, int64_t flank_sites, int64_t n_genes, const int64_t* gene_lo,
+                           const int64_t* gene_hi, int n_bins, const double* edges, int64_t* count,
+                           double* lin_sum, double* log_sum, double* log_sq_sum, double* min_lin, double* min_log,
+                           int64_t* histogram);
+
 /* Forward-only filtering of n_pairs pairs over the full sequence: the
  * filter posterior mean at each site given the sites up to it, [n_sites]
  * [n_pairs] host arrays (lower / upper with GSMC_CI). Same recursion as the

@@ -15,6 +15,8 @@ Contents:
   ``python/gamma_smc_cu/infer.py`` (input coercion, segregating filter,
   auto-theta, default pairs, block sizing) and ``src/bindings.cpp``
   (block windows, blockwise stitching);
+* numpy restatement of the per-gene accumulators of the reference's
+  production caller (``analysis/genome_wide/infer_chromosome.py:157-227``);
 * ``ref_tables`` -- the reference's own ``build_flow_field_cache`` compiled
   from ``/root/reference/src/flow_field.cpp`` into ``oracle/_ref`` (only
   where that tree exists), used to pin ``tables_f32``.
@@ -444,6 +446,63 @@ def site_summary(mean):
     for g in range(G):
         tot = tot + grp[:, g]
     return {"site_mean": (tot / P).astype(np.float32), "site_min": mean.min(axis=1), "site_max": mean.max(axis=1)}
+
+
+GENE_BIN_EDGES = np.linspace(np.log(10.0), np.log(1_000_000.0), 51)  # infer_chromosome.py:52-58
+
+
+def gene_site_indices(positions, genes):
+    """Site indices of each ``(start, end)`` gene: ``start <= position <=
+    end`` (``analysis/genome_wide/infer_chromosome.py:101-112``)."""
+    positions = np.asarray(positions)
+    return [np.where((positions >= s) & (positions <= e))[0].astype(np.int64) for s, e in genes]
+
+
+def gene_pair_values(mean, positions, genes):
+    """Per gene (``None`` below 2 sites), the per-pair float32 values of
+    ``infer_chromosome.py:184-207``: the mean over the gene's sites of
+    ``max(mean, 1)`` and of ``log(max(mean, 1))``, both float32 as numpy
+    computes them there."""
+    mean_safe = np.maximum(np.asarray(mean, np.float32), 1.0)
+    log_mean = np.log(mean_safe)
+    out = []
+    for idxs in gene_site_indices(positions, genes):
+        if idxs.size < 2:
+            out.append(None)
+            continue
+        out.append((mean_safe[idxs, :].mean(axis=0), log_mean[idxs, :].mean(axis=0)))
+    return out
+
+
+def gene_accumulators(mean, positions, genes, edges=GENE_BIN_EDGES, acc=None):
+    """One chunk's update of the per-gene accumulators
+    (``infer_chromosome.py:157-227``): ``mean`` is the chunk's ``(S, P)``
+    float32 posterior mean, ``genes`` ``(start, end)`` bp pairs. Adds into
+    ``acc`` (fresh accumulators when ``None``) and returns it."""
+    edges = np.asarray(edges, np.float64)
+    nb = edges.size - 1
+    n_genes = len(genes)
+    if acc is None:
+        acc = {"count": np.zeros(n_genes, np.int64), "lin_sum": np.zeros(n_genes, np.float64),
+               "log_sum": np.zeros(n_genes, np.float64), "log_sq_sum": np.zeros(n_genes, np.float64),
+               "min_lin": np.full(n_genes, np.inf, np.float64), "min_log": np.full(n_genes, np.inf, np.float64),
+               "histogram": np.zeros((n_genes, nb), np.int64)}
+    n_pairs = np.asarray(mean).shape[1]
+    for gi, v in enumerate(gene_pair_values(mean, positions, genes)):
+        if v is None:
+            continue
+        lin, lg = v
+        acc["count"][gi] += n_pairs
+        acc["lin_sum"][gi] += lin.sum()
+        acc["log_sum"][gi] += lg.sum()
+        acc["log_sq_sum"][gi] += (lg * lg).sum()
+        if lin.min() < acc["min_lin"][gi]:
+            acc["min_lin"][gi] = lin.min()
+        if lg.min() < acc["min_log"][gi]:
+            acc["min_log"][gi] = lg.min()
+        bins = np.clip(np.digitize(lg, edges) - 1, 0, nb - 1)
+        np.add.at(acc["histogram"][gi], bins, 1)
+    return acc
 
 
 def log_error_report(mean32, mean64):
