@@ -28,6 +28,9 @@ The one JSON line also carries:
 * ``biobank``, ``ragged``, ``serving``: compact runs of the other workloads
   (``--workload`` runs any of them alone, at full length), each with its
   own checks;
+* ``gene_stats``: the per-gene accumulators of the reference's production
+  pipeline for all pairs over 800 synthetic genes, computed on the device
+  (only the accumulators come back), checked on sampled pairs;
 * ``plain``: the reference's small one-shot call (190 pairs x 17,339
   sites, ``gamma_smc_flow_cached_fb``) beside its published 0.0333 s;
 * ``scaling``: ``infer()`` over all pairs at the reference's pairwise
@@ -503,8 +506,52 @@ def run_decode_workload(args, name, compact=False):
                                             wl["fp64_pairs"], args.cpu_pairs, args.no_cpu_baseline or compact)
     checks.update(c2)
     out.update({"checks": checks, "cpu_baseline": cpu, "fp64_error": fp64})
+    if not compact and args.gene_steps > 0:
+        out["gene_stats"] = gene_stats_run(ctx, pairs, pos, core, flank, sample, alone, args.gene_steps, wl["span"])
+        checks.update({f"gene_stats_{k}": v for k, v in out["gene_stats"]["checks"].items()})
     state = {"G": G, "pos": pos, "ctx": ctx, "pairs": pairs, "sample": sample, "alone": alone, "rows": rows}
     return out, state
+
+
+def gene_stats_run(ctx, pairs, pos, core, flank, sample, alone, steps, span):
+    """Per-gene statistics of all pairs (``run_fb_gene_stats``: the decode
+    with the gene epilogue in place of row stores, then the per-gene
+    reduction; only the accumulators come back) over 800 synthetic genes,
+    host wall clock per call. Checked on the first 1,000 sampled pairs
+    against the oracle's restatement of the reference's host accumulation
+    over their device means."""
+    from gamma_smc_cu_amd import synthetic
+    from gamma_smc_cu_amd.genes import gene_site_ranges
+    from oracle import oracle as O
+
+    genes = synthetic.genes(span)
+    ranges = gene_site_ranges(pos, genes)
+    P, S = pairs.shape[0], pos.shape[0]
+    ctx.run_fb_gene_stats(pairs, ranges, core_block_sites=core, flank_sites=flank)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        acc = ctx.run_fb_gene_stats(pairs, ranges, core_block_sites=core, flank_sites=flank)
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    covered = np.zeros(S + 1, np.int64)
+    np.add.at(covered, ranges[:, 0], 1)
+    np.add.at(covered, ranges[:, 1], -1)
+    n_cov = int(np.count_nonzero(np.cumsum(covered)[:S] > 0))
+    k = min(1000, len(sample))
+    got = ctx.run_fb_gene_stats(pairs[sample[:k]], ranges, core_block_sites=core, flank_sites=flank)
+    want = O.gene_accumulators(alone[:, :k], pos, genes)
+    sums = all(np.allclose(got[x], want[x], rtol=1e-5, atol=1e-6) for x in ("lin_sum", "log_sum", "log_sq_sum"))
+    mins = all(np.allclose(got[x], want[x], rtol=1e-5, atol=1e-6) for x in ("min_lin", "min_log"))
+    hd = int(np.abs(got["histogram"] - want["histogram"]).sum())
+    checks = {"sample_counts_equal_oracle": bool(np.array_equal(got["count"], want["count"])),
+              "sample_sums_and_minima_within_1e-5": bool(sums and mins),
+              "sample_histogram_moves_le_0.1pct": bool(hd <= 1e-3 * max(int(want["histogram"].sum()), 1))}
+    log(f"gene stats: {ms:.1f} ms/call over {len(genes)} genes ({n_cov} of {S} sites covered), "
+        f"sample histogram moves {hd}")
+    return {"ms": ms, "pair_sites_per_s": float(P) * S / (ms / 1e3), "steps": steps, "n_genes": len(genes),
+            "covered_sites": n_cov, "histogram_bins": int(acc["histogram"].shape[1]),
+            "timing": "host wall clock per call (pair upload, decode with the gene epilogue, per-gene "
+                      "reduction, accumulators back)",
+            "checks": checks}
 
 
 def timed_extras(args, ctx, d_pi, d_pj, d_out, P, S, core, flank, pairs, stream, pair_sites):
@@ -844,6 +891,7 @@ def main():
     ap.add_argument("--codes-steps", type=int, default=2)
     ap.add_argument("--fwd-steps", type=int, default=2)
     ap.add_argument("--summary-steps", type=int, default=2)
+    ap.add_argument("--gene-steps", type=int, default=2)
     ap.add_argument("--no-host-stream", action="store_true")
     ap.add_argument("--copy-calibration-gib", type=float, default=1.0,
                     help="size of the plain copy kernel run after the timed region (counter calibration)")
@@ -923,6 +971,7 @@ def main():
         "fp64_error": head.get("fp64_error"),
         "host_stream": host,
         "site_summary": head.get("site_summary"),
+        "gene_stats": head.get("gene_stats"),
         "forward_only": head.get("forward_only"),
         "codes_u8": head.get("codes_u8"),
         "graph_replay": head.get("graph_replay"),

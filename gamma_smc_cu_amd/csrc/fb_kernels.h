@@ -147,6 +147,8 @@ struct FbArgs {
     double* psum;              // summary launches: per 64-pair group and site, the sum of the means
     float* pmin;               // ... their minimum
     float* pmax;               // ... their maximum
+    const int32_t* gene_piece = nullptr;  // gene launches: code of each site's piece (piece_of_code)
+    double2* gene_part = nullptr;         // ... per piece and pair: (sum of x, sum of log2 x), x = max(mean, 1)
 };
 
 // Lane v of this lane's quad (DPP quad permutation; every lane active). The
@@ -610,6 +612,75 @@ template <int K, int NP> struct FwdStates<K, NP, true> {
     __device__ __forceinline__ void set(int q, int s, float vm, float vc) { col[(q * K + s) * 64] = make_float2(vm, vc); }
 };
 
+// Per-gene epilogue: the lane's running sums over the sites of the current
+// piece (a run of sites inside one block core and one elementary segment of
+// the genes' boundaries; the same piece for the whole wave).
+struct GeneAcc {
+    double sl = 0.0, sg = 0.0;  // sum of x and of log2 x, x = max(mean, 1)
+    int32_t piece = -1;
+};
+
+__device__ __forceinline__ void gene_flush(const FbArgs& a, const JobDev& j, uint32_t col, bool live, GeneAcc& G)
+{
+    if (G.piece >= 0 && live)
+        a.gene_part[(size_t)G.piece * (size_t)j.P + (col >> 2)] = make_double2(G.sl, G.sg);
+    G.sl = G.sg = 0.0;
+}
+
+// Site code: 2 k + 1 for a site of piece k, 2 k for a site in no gene with
+// k pieces before it, so codes never decrease along the sites and a window
+// whose first and last sites share a code lies inside one piece (or one gap).
+__device__ __forceinline__ int32_t piece_of_code(int32_t code) { return (code & 1) ? (code >> 1) : -1; }
+
+__device__ __forceinline__ void gene_site(const FbArgs& a, const JobDev& j, const Lanes<1>& L, int32_t pc, float mean,
+                                          GeneAcc& G)
+{
+    if (pc != G.piece) {
+        gene_flush(a, j, L.col[0], L.live[0], G);
+        G.piece = pc;
+    }
+    if (pc >= 0) {
+        const float x = fmaxf(mean, 1.0f);
+        G.sl += (double)x;
+        G.sg += (double)__log2f(x);
+    }
+}
+
+// The window's sites in decreasing order: with the windows walked last to
+// first this takes every core site in decreasing order, so the sums are the
+// same for any checkpoint interval. A change of piece stores the finished
+// piece's sums (a piece lies inside one block core, so exactly one item
+// stores it per pair). Full windows inside one piece or gap take one test.
+template <int K, bool FAST>
+__device__ __forceinline__ void gene_accumulate(const FbArgs& a, const JobDev& j, const Lanes<1>& L, int64_t ws,
+                                                int64_t cnt, const float (&out)[1][K], GeneAcc& G)
+{
+    if (FAST) {
+        const int32_t c0 = __builtin_amdgcn_readfirstlane(a.gene_piece[ws]);
+        const int32_t c1 = __builtin_amdgcn_readfirstlane(a.gene_piece[ws + K - 1]);
+        if (c0 == c1) {
+            const int32_t pc = piece_of_code(c0);
+            if (pc != G.piece) {
+                gene_flush(a, j, L.col[0], L.live[0], G);
+                G.piece = pc;
+            }
+            if (pc >= 0) {
+#pragma unroll
+                for (int k = K - 1; k >= 0; --k) {
+                    const float x = fmaxf(out[0][k], 1.0f);
+                    G.sl += (double)x;
+                    G.sg += (double)__log2f(x);
+                }
+            }
+            return;
+        }
+    }
+#pragma unroll
+    for (int k = K - 1; k >= 0; --k)
+        if (FAST || k < cnt)
+            gene_site(a, j, L, piece_of_code(__builtin_amdgcn_readfirstlane(a.gene_piece[ws + k])), out[0][k], G);
+}
+
 // Phase-3 step over one checkpoint window, software-pipelined: the
 // backward walk of window w (cnt sites at ws, forward states in F) runs in
 // the same loop as the forward recompute of window w - 1 (K sites) when
@@ -621,12 +692,12 @@ template <int K, int NP> struct FwdStates<K, NP, true> {
 // store (vector-memory counters are shared by loads and stores). FAST (a
 // full window and a full next window, no long gaps in either) has no branch
 // inside the loop, so the compiler can interleave all chains.
-template <int K, int NP, bool CI, bool POST, bool SUM, bool B, bool FAST, bool ST, bool LS>
+template <int K, int NP, bool CI, bool POST, bool SUM, bool B, bool FAST, bool ST, bool LS, bool GENE>
 __device__ __forceinline__ void window_step(const FbArgs& a, const JobDev& j, const Lanes<NP>& L, int64_t cs,
                                             int64_t ws, int64_t cnt, bool has_next, const Window<K, NP>& cur,
                                             const Window<K, NP>& nxt, float (&m)[NP], float (&c)[NP],
                                             FwdStates<K, NP, LS>& F, float (&ym)[NP],
-                                            float (&yc)[NP], float* red)
+                                            float (&yc)[NP], float* red, GeneAcc& G)
 {
     float out[NP][K], oa[NP][K], ob[NP][K];
 #pragma unroll
@@ -679,6 +750,11 @@ __device__ __forceinline__ void window_step(const FbArgs& a, const JobDev& j, co
     }
     if (SUM) {
         summarize<K, NP>(a, j, L, ws, cnt, out, red);
+        return;
+    }
+    if constexpr (GENE) {
+        static_assert(NP == 1, "gene launches decode one pair per lane");
+        gene_accumulate<K, FAST>(a, j, L, ws, cnt, out, G);
         return;
     }
     if (!CI && !POST && a.qbits != 0) {
@@ -879,7 +955,7 @@ __device__ __forceinline__ void forward_item(const FbArgs& a, int item, int lane
 // recompute rides along the previous window's backward walk. ST: the
 // forward states are read back instead of recomputed, so the next
 // window's descriptors and het bits are not needed.
-template <int K, int NP, bool CI, bool POST, bool SUM, bool ST, bool LS = false>
+template <int K, int NP, bool CI, bool POST, bool SUM, bool ST, bool LS = false, bool GENE = false>
 __device__ __forceinline__ void backward_item(const FbArgs& a, int item, int lane, float* red, int slot = -1,
                                               float2* states = nullptr)
 {
@@ -944,6 +1020,7 @@ __device__ __forceinline__ void backward_item(const FbArgs& a, int item, int lan
         }
     }
     bool orient_b = false;
+    GeneAcc G;
     for (;;) {
         const bool has_next = w > 0;
         const int64_t ws = cs + w * K;
@@ -976,24 +1053,25 @@ __device__ __forceinline__ void backward_item(const FbArgs& a, int item, int lan
         const bool fast = has_next && cnt == K && no_long_gaps<K>(cur.d) && (ST || no_long_gaps<K>(nxt.d));
         if (fast) {
             if (orient_b)
-                window_step<K, NP, CI, POST, SUM, true, true, ST, LS>(a, j, L, cs, ws, cnt, true, cur, nxt, m, c, F, ym,
-                                                                      yc, red);
+                window_step<K, NP, CI, POST, SUM, true, true, ST, LS, GENE>(a, j, L, cs, ws, cnt, true, cur, nxt, m, c, F,
+                                                                            ym, yc, red, G);
             else
-                window_step<K, NP, CI, POST, SUM, false, true, ST, LS>(a, j, L, cs, ws, cnt, true, cur, nxt, m, c, F, ym,
-                                                                       yc, red);
+                window_step<K, NP, CI, POST, SUM, false, true, ST, LS, GENE>(a, j, L, cs, ws, cnt, true, cur, nxt, m, c,
+                                                                             F, ym, yc, red, G);
         } else {
             if (orient_b)
-                window_step<K, NP, CI, POST, SUM, true, false, ST, LS>(a, j, L, cs, ws, cnt, has_next, cur, nxt, m, c, F,
-                                                                       ym, yc, red);
+                window_step<K, NP, CI, POST, SUM, true, false, ST, LS, GENE>(a, j, L, cs, ws, cnt, has_next, cur, nxt, m,
+                                                                             c, F, ym, yc, red, G);
             else
-                window_step<K, NP, CI, POST, SUM, false, false, ST, LS>(a, j, L, cs, ws, cnt, has_next, cur, nxt, m, c, F,
-                                                                        ym, yc, red);
+                window_step<K, NP, CI, POST, SUM, false, false, ST, LS, GENE>(a, j, L, cs, ws, cnt, has_next, cur, nxt,
+                                                                              m, c, F, ym, yc, red, G);
         }
         if (!has_next) break;
         orient_b = !orient_b;
         --w;
         cnt = K;
     }
+    if constexpr (GENE) gene_flush(a, j, L.col[0], L.live[0], G);
 }
 
 
@@ -1042,6 +1120,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kBwdWaves<K,
     __shared__ float2 states[kStateLds<K, NP, ST>];
     if (item < a.n_items)
         backward_item<K, NP, CI, POST, SUM, ST, kStatesInLds<K, ST>>(a, item, threadIdx.x, red, -1, states);
+}
+
+// Per-gene launches: the backward walk of fb_bwd_items with the gene
+// epilogue in place of row stores (one pair per lane, static schedule).
+template <int K>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kBwdWaves<K, 1, false>))) void fb_bwd_genes(
+    FbArgs a)
+{
+    const int item = __builtin_amdgcn_readfirstlane((int)blockIdx.x);
+    __shared__ float red[1];
+    __shared__ float2 states[kStateLds<K, 1, false>];
+    if (item < a.n_items)
+        backward_item<K, 1, false, false, false, false, kStatesInLds<K, false>, true>(a, item, threadIdx.x, red, -1,
+                                                                                      states);
 }
 
 // Lane 0 takes the next item of the counter; every caller reaches this with

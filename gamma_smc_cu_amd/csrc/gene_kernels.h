@@ -5,19 +5,14 @@
 // (sites, pairs) means there, per gene: for each pair the mean over the
 // gene's sites of max(mean, 1) and of its natural log, then over the pairs
 // the count, the sums of both and of the squared log, their minima and a
-// histogram of the per-pair log. Here the decoded rows stay on the device
-// and two kernels reduce them:
-//   gene_piece_sums_kernel  per pair and piece (a run of sites inside one
-//                           elementary segment of the genes' boundaries),
-//                           FP64 sums of max(mean, 1) and log2(max(mean, 1))
-//                           in site order;
-//   gene_reduce_kernel      per gene, the pieces it covers summed per pair in
-//                           piece order, then the pairs reduced in a fixed
-//                           tree, accumulated into the running totals of
-//                           earlier pair chunks.
-// Both are streaming kernels: the first reads each covered row once (4 B
-// per pair-site, HBM-bound), the second reads 16 B per pair and covered
-// piece.
+// histogram of the per-pair log. Here the means never leave the decode:
+// the backward kernel's gene epilogue (fb_bwd_genes, fb_kernels.h) sums, per
+// pair and piece (a run of sites inside one block core and one elementary
+// segment of the genes' boundaries), x = max(mean, 1) and log2 x in FP64,
+// and gene_reduce_kernel below turns the piece sums into the per-gene
+// accumulators: per gene, the pieces it covers summed per pair in piece
+// order, then the pairs reduced in a fixed tree and added to the running
+// totals of earlier pair chunks. It reads 16 B per pair and covered piece.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -27,53 +22,8 @@ namespace gsmc {
 
 constexpr int kMaxGeneBins = 1024;  // histogram bins held in LDS per gene
 
-struct GenePieceArgs {
-    const float* mean;         // decoded rows [S][P] (row stride P)
-    int64_t P;
-    const int64_t* piece_lo;   // piece k covers rows [piece_lo[k], piece_hi[k])
-    const int64_t* piece_hi;
-    const int32_t* item_off;   // item i covers pieces [item_off[i], item_off[i + 1])
-    int64_t tiles;             // 64-pair tiles; block b is tile b % tiles of item b / tiles
-    double2* part;             // [n_pieces][P]: (sum of x, sum of log2 x), x = max(mean, 1)
-};
-
-// One wave per (64-pair tile, item). The row loop is unrolled by 8 so that
-// eight loads of the lane's column are in flight at once; the FP64 sums
-// take the sites in increasing order.
-__global__ __launch_bounds__(64) void gene_piece_sums_kernel(GenePieceArgs a)
-{
-    const int64_t b = blockIdx.x;
-    const int64_t tile = b % a.tiles, item = b / a.tiles;
-    const int64_t p = tile * 64 + threadIdx.x;
-    if (p >= a.P) return;
-    const float* col = a.mean + p;
-    const int32_t k0 = a.item_off[item], k1 = a.item_off[item + 1];
-    for (int32_t k = k0; k < k1; ++k) {
-        const int64_t lo = a.piece_lo[k], hi = a.piece_hi[k];
-        double sl = 0.0, sg = 0.0;
-        int64_t s = lo;
-        for (; s + 8 <= hi; s += 8) {
-            float x[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) x[u] = __builtin_nontemporal_load(col + (s + u) * a.P);
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const float v = fmaxf(x[u], 1.0f);
-                sl += (double)v;
-                sg += (double)__log2f(v);
-            }
-        }
-        for (; s < hi; ++s) {
-            const float v = fmaxf(__builtin_nontemporal_load(col + s * a.P), 1.0f);
-            sl += (double)v;
-            sg += (double)__log2f(v);
-        }
-        a.part[(size_t)k * (size_t)a.P + (size_t)p] = make_double2(sl, sg);
-    }
-}
-
 struct GeneReduceArgs {
-    const double2* part;       // [n_pieces][P] from gene_piece_sums_kernel
+    const double2* part;       // [n_pieces][P] piece sums of the gene epilogue
     int64_t P;
     const int32_t* gene_pb;    // gene g covers pieces [gene_pb[g], gene_pe[g])
     const int32_t* gene_pe;

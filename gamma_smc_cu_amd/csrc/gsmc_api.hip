@@ -348,7 +348,7 @@ struct gsmc_context {
     hipEvent_t ev_stage[2] = {nullptr, nullptr};   // staging buffer filled (copy -> host)
     std::unique_ptr<gsmc_batch> dev_batch;
     DevBuf sum_pairs, sum_site;  // per-site summary path: pair chunk, outputs and running values
-    DevBuf gene_pairs, gene_rows, gene_part, gene_meta, gene_acc;  // per-gene statistics path
+    DevBuf gene_pairs, gene_part, gene_meta, gene_acc;  // per-gene statistics path
     void* scratch = nullptr;     // this device's `scratch` (gsmc_run_fb_device_noalloc)
 
     ~gsmc_context();
@@ -530,6 +530,9 @@ struct JobSpec {
     float* tmin = nullptr;
     float* tmax = nullptr;
     int64_t P_total = 0;
+    // Per-gene launches: piece of each site, piece sums out
+    const int32_t* gene_piece = nullptr;
+    double2* gene_part = nullptr;
 };
 
 // Summary partials of one job in a launch.
@@ -571,7 +574,15 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 // SIMD): as two kernels per phase of items (static schedule), or as one
 // fused persistent kernel with a state slot per resident wave (persistent
 // schedule).
-bool stored_states(const Tuning& tu, int flags) { return tu.K == 1 && !(flags & (GSMC_CI | GSMC_POSTERIOR)); }
+// Internal launch flag of per-gene statistics: the backward kernel
+// accumulates per-gene piece sums instead of storing rows (one pair per
+// lane, checkpoint windows, static schedule).
+constexpr int kGeneLaunch = 1 << 8;
+
+bool stored_states(const Tuning& tu, int flags)
+{
+    return tu.K == 1 && !(flags & (GSMC_CI | GSMC_POSTERIOR | kGeneLaunch));
+}
 int window_k(const Tuning& tu) { return tu.K == 8 ? 8 : 16; }
 
 // The tuning one launch runs with: the context's, except that a per-site
@@ -583,6 +594,7 @@ Tuning launch_tuning(const Tuning& t, int flags)
 {
     Tuning u = t;
     if ((flags & GSMC_SUMMARY) && !stored_states(t, flags) && t.K == 8 && t.np == 1) u.schedule = 0;
+    if (flags & kGeneLaunch) u.schedule = 0;
     return u;
 }
 constexpr int kStoredBwdK = 4;
@@ -788,7 +800,7 @@ Plan make_plan(const gsmc_context* ctx, const std::vector<JobSpec>& jobs, const 
 // are built for 1 and 2, the CI / posterior kernels for 1.
 int launch_np(const Tuning& tu, int flags)
 {
-    if (flags & (GSMC_CI | GSMC_POSTERIOR)) return 1;
+    if (flags & (GSMC_CI | GSMC_POSTERIOR | kGeneLaunch)) return 1;
     // stored forward states: one or two pairs per lane with the static
     // schedule, one in the fused persistent kernel
     if (tu.K == 1) return tu.schedule == 1 ? 1 : std::min(tu.np, 2);
@@ -822,6 +834,12 @@ template <int K> KernelPair kernels_k(const Tuning& tu, int flags)
 {
     const bool ci = flags & GSMC_CI, post = flags & GSMC_POSTERIOR;
     const int np = launch_np(tu, flags);
+    if (flags & kGeneLaunch) {
+        // static schedule only (launch_tuning): the items kernels fill every slot
+        const void* f = reinterpret_cast<const void*>(&fb_fwd_items<K, 1, false>);
+        const void* g = reinterpret_cast<const void*>(&fb_bwd_genes<K>);
+        return {f, g, f, g, f, g};
+    }
     if (flags & GSMC_SUMMARY) return np == 2 ? kernels_of<K, 2, false, false, true>() : kernels_of<K, 1, false, false, true>();
     if (ci && post) return kernels_of<K, 1, true, true>();
     if (ci) return kernels_of<K, 1, true, false>();
@@ -985,7 +1003,12 @@ void launch_plan(gsmc_batch* b, const std::vector<JobSpec>& jobs, int flags, hip
         b->last.swap(pl.bytes);
         b->last_stream = stream;
     }
-    const FbArgs a = make_args(b, pl, flags);
+    FbArgs a = make_args(b, pl, flags);
+    if (flags & kGeneLaunch) {
+        require(jobs.size() == 1 && jobs[0].gene_piece && jobs[0].gene_part, "internal: gene launch without pieces");
+        a.gene_piece = jobs[0].gene_piece;
+        a.gene_part = jobs[0].gene_part;
+    }
     enqueue_kernels(ctx, pl, a, flags, stream);
     HIP_OK(hipEventRecord(b->done, stream));
     b->pending = true;
@@ -1686,22 +1709,21 @@ int gsmc_run_fb_summary(gsmc_context* ctx, const int32_t* pi, const int32_t* pj,
 namespace gsmc {
 namespace {
 
-// Work layout of a per-gene statistics call: the genes' boundaries cut the
-// sites into elementary segments; the segments some gene covers are split
-// into pieces of at most kGenePieceSites sites, and consecutive pieces are
-// grouped into items of at most kGeneItemSites sites (one wave per item
-// and 64-pair tile). A gene covers a contiguous run of pieces.
+// Work layout of a per-gene statistics call: the genes' boundaries and the
+// block cores' boundaries cut the sites into pieces; the pieces some gene
+// covers are numbered in site order, and site_piece gives each site a code:
+// 2 k + 1 in piece k, 2 k in no gene after k pieces (non-decreasing along
+// the sites). A gene covers a contiguous run of pieces, and each piece lies
+// inside one block core, so one work item per 64 pairs writes its sums.
 struct GenePlan {
-    std::vector<int64_t> piece_lo, piece_hi;
-    std::vector<int32_t> item_off;
+    std::vector<int32_t> site_piece;
+    int64_t n_pieces = 0;
     std::vector<int32_t> gene_pb, gene_pe;
     std::vector<int64_t> gene_n;
 };
 
-constexpr int64_t kGenePieceSites = 2048;
-constexpr int64_t kGeneItemSites = 4096;
-
-static void plan_genes(int64_t S, int64_t n_genes, const int64_t* lo, const int64_t* hi, GenePlan& gp)
+static void plan_genes(int64_t S, int64_t core, int64_t n_genes, const int64_t* lo, const int64_t* hi,
+                       GenePlan& gp)
 {
     std::vector<std::pair<int64_t, int>> ev;  // (site, +1 gene starts / -1 gene ends)
     ev.reserve((size_t)n_genes * 2);
@@ -1715,38 +1737,35 @@ static void plan_genes(int64_t S, int64_t n_genes, const int64_t* lo, const int6
         }
     }
     std::sort(ev.begin(), ev.end());
+    gp.site_piece.assign((size_t)S, 0);
+    std::vector<int64_t> piece_lo;
+    int64_t done = 0;  // sites [0, done) have their codes
     int cover = 0;
     for (size_t i = 0; i < ev.size();) {
         const int64_t at = ev[i].first;
         while (i < ev.size() && ev[i].first == at) cover += ev[i++].second;
         if (cover > 0 && i < ev.size()) {
-            const int64_t end = ev[i].first;  // next boundary: the segment is [at, end)
-            for (int64_t s = at; s < end; s += kGenePieceSites) {
-                gp.piece_lo.push_back(s);
-                gp.piece_hi.push_back(std::min(end, s + kGenePieceSites));
+            const int64_t end = ev[i].first;  // next boundary: [at, end) is covered
+            for (int64_t s = at; s < end;) {
+                const int64_t e = std::min(end, (s / core + 1) * core);  // up to the block core's end
+                require(piece_lo.size() < (size_t)(INT32_MAX / 2), "too many gene pieces");
+                const int32_t k = (int32_t)piece_lo.size();
+                std::fill(gp.site_piece.begin() + done, gp.site_piece.begin() + s, 2 * k);
+                std::fill(gp.site_piece.begin() + s, gp.site_piece.begin() + e, 2 * k + 1);
+                done = e;
+                piece_lo.push_back(s);
+                s = e;
             }
         }
     }
-    require(gp.piece_lo.size() < (size_t)INT32_MAX, "too many gene pieces");
-    gp.item_off.push_back(0);
-    int64_t run = 0;
-    for (size_t k = 0; k < gp.piece_lo.size(); ++k) {
-        const int64_t len = gp.piece_hi[k] - gp.piece_lo[k];
-        if (run > 0 && run + len > kGeneItemSites) {
-            gp.item_off.push_back((int32_t)k);
-            run = 0;
-        }
-        run += len;
-    }
-    if (!gp.piece_lo.empty()) gp.item_off.push_back((int32_t)gp.piece_lo.size());
+    std::fill(gp.site_piece.begin() + done, gp.site_piece.end(), 2 * (int32_t)piece_lo.size());
+    gp.n_pieces = (int64_t)piece_lo.size();
     gp.gene_pb.resize((size_t)n_genes);
     gp.gene_pe.resize((size_t)n_genes);
     gp.gene_n.resize((size_t)n_genes);
     for (int64_t g = 0; g < n_genes; ++g) {
-        gp.gene_pb[(size_t)g] = (int32_t)(std::lower_bound(gp.piece_lo.begin(), gp.piece_lo.end(), lo[g]) -
-                                          gp.piece_lo.begin());
-        gp.gene_pe[(size_t)g] = (int32_t)(std::lower_bound(gp.piece_lo.begin(), gp.piece_lo.end(), hi[g]) -
-                                          gp.piece_lo.begin());
+        gp.gene_pb[(size_t)g] = (int32_t)(std::lower_bound(piece_lo.begin(), piece_lo.end(), lo[g]) - piece_lo.begin());
+        gp.gene_pe[(size_t)g] = (int32_t)(std::lower_bound(piece_lo.begin(), piece_lo.end(), hi[g]) - piece_lo.begin());
         gp.gene_n[(size_t)g] = hi[g] - lo[g];
     }
 }
@@ -1773,37 +1792,34 @@ int gsmc_run_fb_gene_stats(gsmc_context* ctx, const int32_t* pi, const int32_t* 
                     log_sum != nullptr && log_sq_sum != nullptr && min_lin != nullptr && min_log != nullptr &&
                     histogram != nullptr,
                 "NULL argument");
-        const int64_t S = ctx->S;
-        GenePlan gp;
-        plan_genes(S, n_genes, gene_lo, gene_hi, gp);
-        if (P == 0 || S == 0 || gp.piece_lo.empty()) return;
-        require(pi != nullptr && pj != nullptr, "NULL argument");
         require(n_genes <= INT32_MAX, "too many genes");
+        const int64_t S = ctx->S;
+        const int64_t c = core > 0 ? std::min(core, std::max<int64_t>(S, 1)) : std::max<int64_t>(S, 1);
+        const int64_t fl = core > 0 ? flank : 0;
+        GenePlan gp;
+        plan_genes(S, c, n_genes, gene_lo, gene_hi, gp);
+        if (P == 0 || S == 0 || gp.n_pieces == 0) return;
+        require(pi != nullptr && pj != nullptr, "NULL argument");
         check_pairs(ctx, pi, pj, P);
         std::lock_guard<std::mutex> lk(ctx->mu);
         HIP_OK(hipSetDevice(ctx->device));
         if (!ctx->dev_batch) ctx->dev_batch = new_batch(ctx);
-        const int64_t c = core > 0 ? core : S;
-        const int64_t fl = core > 0 ? flank : 0;
-        const int64_t n_pieces = (int64_t)gp.piece_lo.size();
-        const int64_t n_items = (int64_t)gp.item_off.size() - 1;
-        // Pairs per chunk: decoded rows (4 B per site), piece sums (16 B per
-        // piece) and checkpoints (8 B every K sites) within 40 % of the
-        // memory the call can use (free memory plus the buffers it reuses).
+        const int64_t n_pieces = gp.n_pieces;
+        // Pairs per chunk: piece sums (16 B per piece) and checkpoints (8 B
+        // every K sites of each padded block) within 70 % of the memory the
+        // call can use (free memory plus the buffers it reuses); chunks of
+        // equal size, so that no launch is a small remainder that leaves
+        // most of the chip idle.
         size_t free_b = 0, total_b = 0;
         HIP_OK(hipMemGetInfo(&free_b, &total_b));
-        const double avail = (double)free_b + (double)ctx->dev_batch->ck.bytes + (double)ctx->gene_rows.bytes +
-                             (double)ctx->gene_part.bytes;
-        const double per_pair = 4.0 * (double)S + 16.0 * (double)n_pieces +
-                                8.0 * (double)(S + 2 * fl) / ctx->tune.K + 64.0;
-        int64_t chunk = (int64_t)(0.4 * avail / per_pair) / 64 * 64;
-        chunk = std::max<int64_t>(chunk, 64);
-        chunk = std::min<int64_t>(chunk, (P + 63) / 64 * 64);
-        require((chunk / 64) * n_items <= INT32_MAX, "too many gene work items");
-        // device metadata: pieces, items, genes, edges; accumulators
-        const size_t o_hi = align256((size_t)n_pieces * 8);
-        const size_t o_item = o_hi + align256((size_t)n_pieces * 8);
-        const size_t o_pb = o_item + align256((size_t)(n_items + 1) * 4);
+        const double avail = (double)free_b + (double)ctx->dev_batch->ck.bytes + (double)ctx->gene_part.bytes;
+        const int64_t nb = (S + c - 1) / c;
+        const double per_pair = 16.0 * (double)n_pieces + 8.0 * (double)(S + 2 * fl * nb) / window_k(ctx->tune) + 64.0;
+        const int64_t most = std::max<int64_t>(64, (int64_t)(0.7 * avail / per_pair) / 64 * 64);
+        const int64_t n_chunks = (P + most - 1) / most;
+        const int64_t chunk = ((P + n_chunks - 1) / n_chunks + 63) / 64 * 64;
+        // device metadata: site pieces, genes, edges; accumulators
+        const size_t o_pb = align256((size_t)S * 4);
         const size_t o_pe = o_pb + align256((size_t)n_genes * 4);
         const size_t o_n = o_pe + align256((size_t)n_genes * 4);
         const size_t o_edge = o_n + align256((size_t)n_genes * 8);
@@ -1820,16 +1836,13 @@ int gsmc_run_fb_gene_stats(gsmc_context* ctx, const int32_t* pi, const int32_t* 
         ctx->gene_meta.ensure(meta_bytes, ctx->device, true);
         ctx->gene_acc.ensure(acc_bytes, ctx->device, true);
         ctx->gene_pairs.ensure((size_t)chunk * 2 * sizeof(int32_t), ctx->device, true);
-        ctx->gene_rows.ensure((size_t)chunk * (size_t)S * sizeof(float), ctx->device, true);
         ctx->gene_part.ensure((size_t)chunk * (size_t)n_pieces * sizeof(double2), ctx->device, true);
         char* meta = ctx->gene_meta.as<char>();
         char* acc = ctx->gene_acc.as<char>();
         auto h2d = [&](void* dst, const void* src, size_t bytes) {
             if (bytes) HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->compute));
         };
-        h2d(meta, gp.piece_lo.data(), (size_t)n_pieces * 8);
-        h2d(meta + o_hi, gp.piece_hi.data(), (size_t)n_pieces * 8);
-        h2d(meta + o_item, gp.item_off.data(), (size_t)(n_items + 1) * 4);
+        h2d(meta, gp.site_piece.data(), (size_t)S * 4);
         h2d(meta + o_pb, gp.gene_pb.data(), (size_t)n_genes * 4);
         h2d(meta + o_pe, gp.gene_pe.data(), (size_t)n_genes * 4);
         h2d(meta + o_n, gp.gene_n.data(), (size_t)n_genes * 8);
@@ -1842,14 +1855,8 @@ int gsmc_run_fb_gene_stats(gsmc_context* ctx, const int32_t* pi, const int32_t* 
         h2d(acc + a_mlog, min_log, (size_t)n_genes * 8);
         h2d(acc + a_hist, histogram, (size_t)n_genes * (size_t)n_bins * 8);
 
-        GenePieceArgs ga;
-        ga.piece_lo = reinterpret_cast<const int64_t*>(meta);
-        ga.piece_hi = reinterpret_cast<const int64_t*>(meta + o_hi);
-        ga.item_off = reinterpret_cast<const int32_t*>(meta + o_item);
-        ga.mean = ctx->gene_rows.as<float>();
-        ga.part = ctx->gene_part.as<double2>();
         GeneReduceArgs gr;
-        gr.part = ga.part;
+        gr.part = ctx->gene_part.as<double2>();
         gr.gene_pb = reinterpret_cast<const int32_t*>(meta + o_pb);
         gr.gene_pe = reinterpret_cast<const int32_t*>(meta + o_pe);
         gr.gene_n = reinterpret_cast<const int64_t*>(meta + o_n);
@@ -1876,13 +1883,9 @@ int gsmc_run_fb_gene_stats(gsmc_context* ctx, const int32_t* pi, const int32_t* 
             js.pi = d_pi;
             js.pj = d_pj;
             js.P = n;
-            js.out.mean = ctx->gene_rows.as<float>();
-            launch_plan(ctx->dev_batch.get(), {js}, 0, ctx->compute);
-            ga.P = n;
-            ga.tiles = (n + 63) / 64;
-            hipLaunchKernelGGL(gene_piece_sums_kernel, dim3((unsigned)(ga.tiles * n_items)), dim3(64), 0,
-                               ctx->compute, ga);
-            HIP_OK(hipGetLastError());
+            js.gene_piece = reinterpret_cast<const int32_t*>(meta);
+            js.gene_part = ctx->gene_part.as<double2>();
+            launch_plan(ctx->dev_batch.get(), {js}, kGeneLaunch, ctx->compute);
             gr.P = n;
             hipLaunchKernelGGL(gene_reduce_kernel, dim3((unsigned)n_genes), dim3(256), 0, ctx->compute, gr);
             HIP_OK(hipGetLastError());

@@ -5,8 +5,9 @@ infer_chromosome.py:140-227``) decodes a population's pairs in chunks with
 ``infer_blockwise``, brings each chunk's ``(n_sites, n_pairs)`` means to the
 host and reduces them there, per gene, into running accumulators (written
 out as ``:259-276``). :func:`gene_stats` returns the same accumulators while
-the means never leave the device (``FlowContext.run_fb_gene_stats``, one
-decode and two reduction kernels per pair chunk).
+the means are never stored (``FlowContext.run_fb_gene_stats``: the decode's
+backward kernel sums each pair's values per gene piece in place of writing
+rows, and one more kernel reduces the pieces per gene).
 
 Per gene ``g`` with ``n_g >= 2`` sites ``(positions >= start) & (positions
 <= end)`` and each pair: ``lin`` = mean over the gene's sites of

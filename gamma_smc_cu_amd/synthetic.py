@@ -108,6 +108,19 @@ def ragged_jobs(n_haps, n_sites, n_jobs=64, seed=11, sites=(2_000, 200_000), pai
     return out
 
 
+def genes(span, n_genes=800, seed=29, length=(1_000, 200_000)):
+    """Synthetic gene intervals on ``[0, span)`` bp for per-gene statistics:
+    gene k draws, from ``numpy.random.default_rng([seed, 2])``, a length
+    ``exp(U(log lo, log hi))`` bp and a start uniform among those that fit.
+    Genes may overlap, as annotated genes do. Returns ``(start, end)``
+    integer pairs (``end`` inclusive), sorted by start."""
+    rng = np.random.default_rng([seed, 2])
+    ln = np.exp(rng.uniform(np.log(length[0]), np.log(length[1]), n_genes)).astype(np.int64)
+    st = (rng.random(n_genes) * np.maximum(span - ln, 1)).astype(np.int64)
+    order = np.argsort(st, kind="stable")
+    return [(int(st[k]), int(st[k] + ln[k] - 1)) for k in order]
+
+
 def resident_cohort(n, S, span, seed, chunk=2048, threads=None):
     """Bitpacked 1/k-spectrum cohort for large ``n`` (the serving cohort):
     ``(packed uint64 (n, W), positions float64 (S,))``.

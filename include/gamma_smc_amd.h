@@ -224,7 +224,8 @@ int gsmc_run_fb_summary(gsmc_context* ctx, const int32_t* pair_i, const int32_t*
  * values, n_bins <= 1024). All outputs accumulate, so a caller covering
  * the pairs in several calls initialises them once (0, 0, 0, 0, +inf,
  * +inf, 0). Decodes blockwise as gsmc_run_fb_blockwise (core_block_sites
- * <= 0: the full sequence) into device rows and reduces them there.
+ * <= 0: the full sequence); the backward kernel sums each pair's values per
+ * gene piece in place of storing rows, so no per-pair means are written.
  * Replaces the per-chunk host accumulation of the reference's per-gene
  * pipeline (analysis/genome_wide/infer_chromosome.py:157-227), which reads
  * each chunk's (sites, pairs) means back from infer_blockwise first. */
