@@ -68,11 +68,11 @@ REFERENCE_E2E_SUBSETS = [(1, 1.1561), (10, 1.2289), (100, 1.3497), (1000, 1.6851
 # counter summary under profiles/ gives the commands. Any other kernel
 # source reports traffic null.
 MEASURED_TRAFFIC = {
-    "kernel_sha256": "a1ae1fc2ce959aa289ac6640cd319b569be20e449d9900c3da306b384ef10762",
+    "kernel_sha256": "5f74bb52d1d56b2fc3203d81c7af9bc329c0c5199468dd396a29812a0a84e91c",
     "entries": {
-        "chromosome/K8/np1/dynamic/xcd0": 190.33e9,
-        "chromosome/K8/np1/persistent/xcd0": 186.56e9,
-        "chromosome/K8/np1/static/xcd0": 197.63e9,
+        "chromosome/K8/np1/dynamic/xcd0": 195.32e9,
+        "chromosome/K8/np1/persistent/xcd0": 189.00e9,
+        "chromosome/K8/np1/static/xcd0": 197.57e9,
     },
 }
 
