@@ -154,6 +154,32 @@ def test_single_site_and_empty(core, flow_field_path):
     assert r["mean"].shape == (1, 0) and r["blocks"].shape == (1, 4)
 
 
+def test_pair_symmetry_self_pairs_and_two_haplotypes(core, cohort_small, flow_field_path):
+    """A pair and its swap decode to the same bits (het calls are the XOR of
+    the two rows); a self pair has no het site and equals the oracle; a
+    two-haplotype cohort has its single pair. The reference's property
+    suite checks the same symmetry and edge cases on its HMM path
+    (tests/property/test_invariants.py:17-34, test_edge_cases.py:68-118)."""
+    G, pos = cohort_small
+    ctx = core.FlowContext(G, pos, 1e4, 1.25e-8, 1e-8, flow_field_path, 0)
+    pairs = [(0, 1), (1, 0), (3, 3), (2, 7), (7, 2), (11, 0), (0, 11)]
+    _case("pair symmetry")
+    got = ctx.run_fb_blockwise(pairs, core_block_sites=200, flank_sites=64)["mean"]
+    for a, b in ((0, 1), (3, 4), (5, 6)):
+        assert _bits_equal(got[:, a], got[:, b]), (pairs[a], pairs[b])
+    ref = O.fb_blockwise_f32(G, pos, [(3, 3)], _tables(flow_field_path, 1.25e-8, 1e-8), 1e4, 200, 64)["mean"]
+    assert _bits_equal(got[:, 2:3], ref)
+    _case("two haplotypes")
+    G2 = np.ascontiguousarray(G[[4, 9]])
+    keep = G2.any(axis=0) & ~G2.all(axis=0)
+    G2, pos2 = np.ascontiguousarray(G2[:, keep]), pos[keep]
+    ctx2 = core.FlowContext(G2, pos2, 1e4, 1.25e-8, 1e-8, flow_field_path, 0)
+    got2 = ctx2.run_fb([(1, 0)], mean_only=False)
+    ref2 = O.fb_window_f32(G2, pos2, [(1, 0)], _tables(flow_field_path, 1.25e-8, 1e-8), 1e4, ci=True)
+    for k, v in ref2.items():
+        assert _bits_equal(got2[k], v), k
+
+
 def test_pair_index_out_of_range_raises(core, cohort_small, flow_field_path):
     G, pos = cohort_small
     ctx = core.FlowContext(G, pos, 1e4, 1.25e-8, 1e-8, flow_field_path, 0)
