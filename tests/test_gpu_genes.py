@@ -103,9 +103,32 @@ def test_gene_stats_accumulate_over_pair_chunks(core, cohort_small, flow_field_p
         assert np.allclose(one[k], two[k], rtol=1e-12, atol=0), k
 
 
+def test_gene_stats_same_under_every_tuning(core, cohort_small, flow_field_path):
+    """The gene epilogue sums each piece in decreasing site order whatever
+    the checkpoint interval, and gene launches run one pair per lane on the
+    static schedule whatever the tuning: every field is bit-identical."""
+    from gamma_smc_cu_amd.genes import gene_site_ranges
+
+    G, pos = cohort_small
+    pairs = O.all_pairs(G.shape[0])
+    r = gene_site_ranges(pos, _mixed_genes(pos))
+    ctx = core.FlowContext(G, pos, 1e4, 1.25e-8, 1e-8, flow_field_path, 0)
+    base = None
+    for k, np_, sch in ((8, 1, "static"), (16, 2, "persistent"), (1, 1, "persistent"), (8, 4, "dynamic")):
+        _case(f"gene stats tuning K={k} np={np_} {sch}")
+        ctx.set_tuning(checkpoint_k=k, pairs_per_lane=np_, schedule=sch)
+        got = ctx.run_fb_gene_stats(pairs, r, core_block_sites=200, flank_sites=64)
+        if base is None:
+            base = got
+            continue
+        for f in ("count", "lin_sum", "log_sum", "log_sq_sum", "min_lin", "min_log", "histogram"):
+            assert np.array_equal(got[f], base[f]), (k, np_, sch, f)
+
+
 def test_gene_stats_long_genes(core, flow_field_path):
-    """9,000 sites: genes longer than a piece (2,048 sites) and an item
-    (4,096 sites), pieces grouped into items, three blocks."""
+    """9,000 sites in three block cores: genes spanning one, two and three
+    cores (their pieces split at the cores' boundaries), genes ending on a
+    core boundary, short genes at both ends."""
     from gamma_smc_cu_amd import synthetic
     from gamma_smc_cu_amd.genes import gene_site_ranges
 
@@ -114,7 +137,7 @@ def test_gene_stats_long_genes(core, flow_field_path):
     T = O.tables_f32(flow_field_path, 1000, 1e4, mu, rho)
     pairs = O.all_pairs(8)
     p = pos
-    genes = [(p[0], p[-1]), (p[100], p[8500]), (p[2047], p[2049]), (p[4000], p[4100]), (p[4095], p[8191]),
+    genes = [(p[0], p[-1]), (p[100], p[8500]), (p[2999], p[3000]), (p[4000], p[4100]), (p[4095], p[5999]),
              (p[10], p[12]), (p[8990], p[8999])]
     _case("gene stats, 8 x 9000, core 3000")
     ctx = core.FlowContext(G, pos, 1e4, mu, rho, flow_field_path, 0)
