@@ -471,8 +471,17 @@ class FlowContext:
         if edges.ndim != 1 or edges.size < 2:
             raise ValueError("bin_edges must hold at least two edges")
         acc = new_accumulators(gs.shape[0], edges) if accumulators is None else accumulators
-        if acc["histogram"].shape != (gs.shape[0], edges.size - 1):
-            raise ValueError("accumulators do not match the genes and bins")
+        want = {"count": (np.int64, (gs.shape[0],)), "lin_sum": (np.float64, (gs.shape[0],)),
+                "log_sum": (np.float64, (gs.shape[0],)), "log_sq_sum": (np.float64, (gs.shape[0],)),
+                "min_lin": (np.float64, (gs.shape[0],)), "min_log": (np.float64, (gs.shape[0],)),
+                "histogram": (np.int64, (gs.shape[0], edges.size - 1))}
+        for k, (dt, shape) in want.items():
+            a = acc.get(k)
+            # the library adds into these arrays in place
+            if not (isinstance(a, np.ndarray) and a.dtype == dt and a.shape == shape and a.flags.c_contiguous
+                    and a.flags.writeable):
+                raise ValueError(f"accumulators[{k!r}] must be a writeable C-contiguous {np.dtype(dt).name} "
+                                 f"array of shape {shape} (the genes and bins of this call)")
         if flank_sites < 0:
             raise ValueError("flank_sites must be non-negative.")
         pi, pj = _split_pairs(pairs)

@@ -136,3 +136,21 @@ def test_accumulators_and_summary():
     assert s["arith_mean_tmrca"][1:] == pytest.approx([100.0, 7.0])
     custom = GS.new_accumulators(2, np.linspace(0.0, 1.0, 5))
     assert custom["histogram"].shape == (2, 4)
+
+
+def test_run_fb_gene_stats_checks_accumulators_before_the_library():
+    """The library adds into the accumulator arrays in place, so arrays of
+    another dtype, shape or layout are refused before any native call."""
+    from gamma_smc_cu_amd import _core
+
+    ctx = _core.FlowContext.__new__(_core.FlowContext)   # no device needed: the check comes first
+    ranges = [(0, 5), (1, 3)]
+    for key, bad in (("lin_sum", lambda a: a.astype(np.float32)), ("count", lambda a: a[:1]),
+                     ("histogram", lambda a: np.asfortranarray(np.zeros((2, 50), np.int64))),
+                     ("min_log", lambda a: np.broadcast_to(a, a.shape))):
+        acc = GS.new_accumulators(2)
+        acc[key] = bad(acc[key])
+        with pytest.raises(ValueError, match=key):
+            ctx.run_fb_gene_stats([(0, 1)], ranges, accumulators=acc)
+    with pytest.raises(ValueError, match="histogram"):
+        ctx.run_fb_gene_stats([(0, 1)], ranges, accumulators=GS.new_accumulators(2, np.linspace(0, 1, 4)))
