@@ -122,8 +122,8 @@ def _declare(L):
         "gsmc_batch_reserve": [_vp, ctypes.c_uint64, _i64, _i64],
         "gsmc_batch_buffer_bytes": [_vp, ctypes.POINTER(ctypes.c_uint64)],
         "gsmc_run_fb_summary": [_vp, _vp, _vp, _i64, _i64, _i64, _vp, _vp, _vp],
-        "gsmc_run_fb_gene_stats": [_vp, _vp, _vp, _i64, _i64, _i64, _i64, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp,
-                                   _vp, _vp, _vp],
+        "gsmc_run_fb_gene_stats": [_vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64, _vp, _vp, _i32, _vp, _vp, _vp, _vp,
+                                   _vp, _vp, _vp, _vp],
         "gsmc_run_fwd": [_vp, _vp, _vp, _i64, _i32, _vp, _vp, _vp],
         "gsmc_run_fwd_device": [_vp, _vp, _vp, _i64, _i32, _vp, _vp, _vp, _vp],
         "gsmc_batch_capture": [_vp, _vp, _i64, _i32, ctypes.POINTER(_vp)],
@@ -454,7 +454,7 @@ class FlowContext:
         return out
 
     def run_fb_gene_stats(self, pairs, gene_sites, core_block_sites=0, flank_sites=0, bin_edges=None,
-                          accumulators=None):
+                          accumulators=None, pair_batch_size=-1):
         """Per-gene statistics of the posterior means over ``pairs``, without
         per-pair outputs: the accumulators of the reference's per-gene
         pipeline (``analysis/genome_wide/infer_chromosome.py:157-227``) for
@@ -463,7 +463,8 @@ class FlowContext:
         ``log_sum``, ``log_sq_sum``, ``min_lin``, ``min_log``, ``histogram``
         and ``bin_edges``; pass the result back as ``accumulators`` to add
         further pairs to it. ``core_block_sites <= 0`` decodes the full
-        sequence."""
+        sequence; ``pair_batch_size`` caps the pairs per launch (-1: as many
+        as device memory holds)."""
         from .genes import DEFAULT_BIN_EDGES, new_accumulators
 
         gs = np.ascontiguousarray(np.asarray(gene_sites, dtype=np.int64).reshape(-1, 2))
@@ -486,8 +487,11 @@ class FlowContext:
             raise ValueError("flank_sites must be non-negative.")
         pi, pj = _split_pairs(pairs)
         lo, hi = np.ascontiguousarray(gs[:, 0]), np.ascontiguousarray(gs[:, 1])
+        if pair_batch_size == 0 or pair_batch_size < -1:
+            raise ValueError("pair_batch_size must be positive or -1 for auto.")
         check(lib().gsmc_run_fb_gene_stats(self._h, _ptr(pi), _ptr(pj), pi.shape[0], int(core_block_sites),
-                                           int(flank_sites), gs.shape[0], _ptr(lo), _ptr(hi), edges.size - 1,
+                                           int(flank_sites), int(pair_batch_size), gs.shape[0], _ptr(lo), _ptr(hi),
+                                           edges.size - 1,
                                            _ptr(edges), _ptr(acc["count"]), _ptr(acc["lin_sum"]),
                                            _ptr(acc["log_sum"]), _ptr(acc["log_sq_sum"]), _ptr(acc["min_lin"]),
                                            _ptr(acc["min_log"]), _ptr(acc["histogram"])))

@@ -1773,7 +1773,8 @@ static void plan_genes(int64_t S, int64_t core, int64_t n_genes, const int64_t* 
 }  // namespace gsmc
 
 int gsmc_run_fb_gene_stats(gsmc_context* ctx, const int32_t* pi, const int32_t* pj, int64_t P, int64_t core,
-                           int64_t flank, int64_t n_genes, const int64_t* gene_lo, const int64_t* gene_hi, int n_bins,
+                           int64_t flank, int64_t pair_batch, int64_t n_genes, const int64_t* gene_lo,
+                           const int64_t* gene_hi, int n_bins,
                            const double* edges, int64_t* count, double* lin_sum, double* log_sum, double* log_sq_sum,
                            double* min_lin, double* min_log, int64_t* histogram)
 {
@@ -1815,9 +1816,11 @@ int gsmc_run_fb_gene_stats(gsmc_context* ctx, const int32_t* pi, const int32_t* 
         const double avail = (double)free_b + (double)ctx->dev_batch->ck.bytes + (double)ctx->gene_part.bytes;
         const int64_t nb = (S + c - 1) / c;
         const double per_pair = 16.0 * (double)n_pieces + 8.0 * (double)(S + 2 * fl * nb) / window_k(ctx->tune) + 64.0;
-        const int64_t most = std::max<int64_t>(64, (int64_t)(0.7 * avail / per_pair) / 64 * 64);
+        int64_t most = std::max<int64_t>(64, (int64_t)(0.7 * avail / per_pair) / 64 * 64);
+        if (pair_batch > 0) most = std::min(most, pair_batch);
         const int64_t n_chunks = (P + most - 1) / most;
-        const int64_t chunk = ((P + n_chunks - 1) / n_chunks + 63) / 64 * 64;
+        const int64_t per = (P + n_chunks - 1) / n_chunks;
+        const int64_t chunk = pair_batch > 0 ? per : (per + 63) / 64 * 64;
         // device metadata: site pieces, genes, edges; accumulators
         const size_t o_pb = align256((size_t)S * 4);
         const size_t o_pe = o_pb + align256((size_t)n_genes * 4);

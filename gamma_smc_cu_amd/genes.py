@@ -82,7 +82,7 @@ def gene_site_ranges(positions, genes):
 
 def gene_stats(G_or_ts, positions=None, genes=(), mu=1.25e-8, rho=1e-8, Ne=10000, pairs=None,
                flow_field_path=None, core_block_sites="auto", flank_sites=2048, auto_estimate_theta=True,
-               bin_edges=None, accumulators=None):
+               bin_edges=None, accumulators=None, pair_batch_size=-1):
     """Per-gene accumulators of the posterior means over ``pairs`` (default:
     all pairs ``i < j`` in the reference pipeline's order, ``make_pairs``,
     ``infer_chromosome.py:97-98``). Inputs are prepared as
@@ -90,7 +90,8 @@ def gene_stats(G_or_ts, positions=None, genes=(), mu=1.25e-8, rho=1e-8, Ne=10000
     ``core_block_sites="auto"`` sized from free device memory), and genes
     are matched against the kept sites' positions, as the reference matches
     them against ``result["positions"]``. Pass an earlier result as
-    ``accumulators`` to add more pairs to it.
+    ``accumulators`` to add more pairs to it; ``pair_batch_size`` caps the
+    pairs per launch (-1: as many as device memory holds).
 
     Returns the accumulators and ``n_sites_per_gene`` (int32), ``n_pairs``
     (this call) and ``positions`` (the kept sites)."""
@@ -128,7 +129,7 @@ def gene_stats(G_or_ts, positions=None, genes=(), mu=1.25e-8, rho=1e-8, Ne=10000
         ctx = _core.FlowContext(G, positions, float(Ne), kmu, krho, flow_field_path, 0)
         try:
             ctx.run_fb_gene_stats(pairs, ranges, core_block_sites=int(core_block_sites), flank_sites=flank_sites,
-                                  bin_edges=acc["bin_edges"], accumulators=acc)
+                                  bin_edges=acc["bin_edges"], accumulators=acc, pair_batch_size=pair_batch_size)
         finally:
             ctx.close()
     acc["n_sites_per_gene"] = (ranges[:, 1] - ranges[:, 0]).astype(np.int32)

@@ -226,11 +226,15 @@ int gsmc_run_fb_summary(gsmc_context* ctx, const int32_t* pair_i, const int32_t*
  * +inf, 0). Decodes blockwise as gsmc_run_fb_blockwise (core_block_sites
  * <= 0: the full sequence); the backward kernel sums each pair's values per
  * gene piece in place of storing rows, so no per-pair means are written.
+ * Pairs go in launches of at most pair_batch (<= 0: as many as 70 % of the
+ * device memory holds); results do not depend on it beyond the order of
+ * the FP64 sums over pairs.
  * Replaces the per-chunk host accumulation of the reference's per-gene
  * pipeline (analysis/genome_wide/infer_chromosome.py:157-227), which reads
  * each chunk's (sites, pairs) means back from infer_blockwise first. */
 int gsmc_run_fb_gene_stats(gsmc_context* ctx, const int32_t* pair_i, const int32_t* pair_j, int64_t n_pairs,
-                           int64_t core_block_sites, int64_t flank_sites, int64_t n_genes, const int64_t* gene_lo,
+                           int64_t core_block_sites, int64_t flank_sites, int64_t pair_batch, int64_t n_genes,
+                           const int64_t* gene_lo,
                            const int64_t* gene_hi, int n_bins, const double* edges, int64_t* count,
                            double* lin_sum, double* log_sum, double* log_sq_sum, double* min_lin, double* min_log,
                            int64_t* histogram);

@@ -83,9 +83,10 @@ def test_gene_stats_match_oracle(core, cohort_small, flow_field_path, csz, fl):
 
 def test_gene_stats_accumulate_over_pair_chunks(core, cohort_small, flow_field_path):
     """Two calls over halves of the pairs, the second adding to the first's
-    accumulators, give the one-call result: per-pair values do not depend on
-    the chunk, so counts, minima and histograms are equal and the sums
-    differ only by the order of the float64 additions."""
+    accumulators, and one call in launches of 20 pairs give the one-launch
+    result: per-pair values do not depend on the chunk, so counts, minima
+    and histograms are equal and the sums differ only by the order of the
+    float64 additions."""
     from gamma_smc_cu_amd.genes import gene_site_ranges
 
     G, pos = cohort_small
@@ -97,10 +98,15 @@ def test_gene_stats_accumulate_over_pair_chunks(core, cohort_small, flow_field_p
     _case("gene stats, two chunks")
     two = ctx.run_fb_gene_stats(pairs[:30], r, core_block_sites=200, flank_sites=64)
     two = ctx.run_fb_gene_stats(pairs[30:], r, core_block_sites=200, flank_sites=64, accumulators=two)
-    for k in ("count", "min_lin", "min_log", "histogram"):
-        assert np.array_equal(one[k], two[k]), k
-    for k in ("lin_sum", "log_sum", "log_sq_sum"):
-        assert np.allclose(one[k], two[k], rtol=1e-12, atol=0), k
+    _case("gene stats, launches of 20 pairs")
+    four = ctx.run_fb_gene_stats(pairs, r, core_block_sites=200, flank_sites=64, pair_batch_size=20)
+    for other in (two, four):
+        for k in ("count", "min_lin", "min_log", "histogram"):
+            assert np.array_equal(one[k], other[k]), k
+        for k in ("lin_sum", "log_sum", "log_sq_sum"):
+            assert np.allclose(one[k], other[k], rtol=1e-12, atol=0), k
+    with pytest.raises(ValueError, match="pair_batch_size"):
+        ctx.run_fb_gene_stats(pairs, r, pair_batch_size=0)
 
 
 def test_gene_stats_same_under_every_tuning(core, cohort_small, flow_field_path):
