@@ -579,6 +579,14 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 // lane, checkpoint windows, static schedule).
 constexpr int kGeneLaunch = 1 << 8;
 
+// Flags a caller may pass (the GSMC_* run flags); other bits, the internal
+// launch flags among them, are refused.
+constexpr int kPublicFlags = GSMC_CI | GSMC_POSTERIOR | GSMC_SUMMARY | GSMC_Q8 | GSMC_Q4;
+void check_flags(int flags)
+{
+    require((flags & ~kPublicFlags) == 0, "unknown flag bits: " + std::to_string(flags & ~kPublicFlags));
+}
+
 bool stored_states(const Tuning& tu, int flags)
 {
     return tu.K == 1 && !(flags & (GSMC_CI | GSMC_POSTERIOR | kGeneLaunch));
@@ -1469,6 +1477,7 @@ int gsmc_run_fb(gsmc_context* ctx, const int32_t* pi, const int32_t* pj, int64_t
                 float* lower, float* upper, float* alpha, float* beta)
 {
     return guarded([&] {
+        check_flags(flags);
         require(ctx != nullptr, "context is NULL");
         require(P >= 0, "negative n_pairs");
         if (P == 0 || ctx->S == 0) return;
@@ -1493,6 +1502,7 @@ int gsmc_run_fb_blockwise(gsmc_context* ctx, const int32_t* pi, const int32_t* p
                           float* upper, float* alpha, float* beta)
 {
     return guarded([&] {
+        check_flags(flags);
         require(ctx != nullptr, "context is NULL");
         require(core > 0, "core_block_sites must be positive.");
         require(flank >= 0, "flank_sites must be non-negative.");
@@ -1524,6 +1534,7 @@ int gsmc_run_fb_device(gsmc_context* ctx, const int32_t* d_pi, const int32_t* d_
                        float* d_beta, void* stream)
 {
     return guarded([&] {
+        check_flags(flags);
         require(ctx != nullptr, "context is NULL");
         require(core > 0, "core_block_sites must be positive.");
         require(flank >= 0, "flank_sites must be non-negative.");
@@ -1552,6 +1563,7 @@ int gsmc_run_fb_device_noalloc(gsmc_context* ctx, const int32_t* d_pi, const int
                                float* d_beta, void* stream)
 {
     return guarded([&] {
+        check_flags(flags);
         require(ctx != nullptr, "context is NULL");
         require(core > 0, "core_block_sites must be positive.");
         require(flank >= 0, "flank_sites must be non-negative.");
@@ -1941,6 +1953,7 @@ int gsmc_run_fwd_device(gsmc_context* ctx, const int32_t* d_pi, const int32_t* d
                         float* d_mean, float* d_lower, float* d_upper, void* stream)
 {
     return guarded([&] {
+        check_flags(flags);
         require(ctx != nullptr, "context is NULL");
         require(P >= 0, "negative n_pairs");
         if (P == 0 || ctx->S == 0) return;
@@ -1956,6 +1969,7 @@ int gsmc_run_fwd(gsmc_context* ctx, const int32_t* pi, const int32_t* pj, int64_
                  float* lower, float* upper)
 {
     return guarded([&] {
+        check_flags(flags);
         require(ctx != nullptr, "context is NULL");
         require(P >= 0, "negative n_pairs");
         if (P == 0 || ctx->S == 0) return;
@@ -2142,6 +2156,7 @@ struct gsmc_graph {
 int gsmc_batch_capture(gsmc_context* ctx, const gsmc_job* jobs, int64_t n_jobs, int flags, gsmc_graph** out)
 {
     return guarded([&] {
+        check_flags(flags);
         require(ctx != nullptr && out != nullptr, "NULL argument");
         require(n_jobs >= 0 && (jobs != nullptr || n_jobs == 0), "bad job list");
         require(!((flags & GSMC_SUMMARY) && (flags & (GSMC_CI | GSMC_POSTERIOR))),
@@ -2235,6 +2250,7 @@ std::vector<JobSpec> specs_of(const gsmc_context* ctx, const gsmc_job* jobs, int
 int gsmc_batch_launch(gsmc_batch* b, const gsmc_job* jobs, int64_t n_jobs, int flags, void* stream)
 {
     return guarded([&] {
+        check_flags(flags);
         require(b != nullptr, "batch is NULL");
         require(n_jobs >= 0 && (jobs != nullptr || n_jobs == 0), "bad job list");
         gsmc_context* ctx = b->ctx;

@@ -328,3 +328,19 @@ class TestCABI:
             _core.build_tables("/nonexistent/field.txt", 10, 1e4, 1e-8, 1e-8)
         with pytest.raises(ValueError, match="cache_steps"):
             _core.build_tables(infer_mod._resolve_flow_field_path(None), 0, 1e4, 1e-8, 1e-8)
+
+    def test_unknown_flag_bits_are_refused_before_any_device_call(self):
+        """Entry points taking run flags refuse bits outside GSMC_CI ..
+        GSMC_Q4 (the library's internal launch flags included) first, so
+        the call fails with GSMC_EINVAL and a message without touching a
+        device."""
+        from gamma_smc_cu_amd import _core
+
+        L = _core.lib()
+        for bit in (32, 256, 1 << 20):
+            assert L.gsmc_run_fb(None, None, None, 0, bit, None, None, None, None, None) == -1
+            assert b"unknown flag bits" in L.gsmc_last_error()
+            assert L.gsmc_run_fb_device(None, None, None, 0, 8, 0, bit, None, None, None, None, None, None) == -1
+            assert L.gsmc_batch_launch(None, None, 0, bit, None) == -1
+            assert L.gsmc_run_fwd(None, None, None, 0, bit, None, None, None) == -1
+            assert b"unknown flag bits" in L.gsmc_last_error()
