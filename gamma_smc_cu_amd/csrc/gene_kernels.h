@@ -9,10 +9,12 @@
 // the backward kernel's gene epilogue (fb_bwd_genes, fb_kernels.h) sums, per
 // pair and piece (a run of sites inside one block core and one elementary
 // segment of the genes' boundaries), x = max(mean, 1) and log2 x in FP64,
-// and gene_reduce_kernel below turns the piece sums into the per-gene
-// accumulators: per gene, the pieces it covers summed per pair in piece
-// order, then the pairs reduced in a fixed tree and added to the running
-// totals of earlier pair chunks. It reads 16 B per pair and covered piece.
+// and gene_reduce_kernel / gene_combine_kernel below turn the piece sums
+// into the per-gene accumulators: per gene, the pieces it covers summed per
+// pair in piece order, the pairs reduced in splits (several workgroups per
+// gene, so that few genes still fill the chip) by a fixed tree, and the
+// splits added in order to the running totals of earlier pair chunks. The
+// reduction reads 16 B per pair and covered piece.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -37,19 +39,24 @@ struct GeneReduceArgs {
     double* min_lin;
     double* min_log;
     int64_t* hist;             // [n_genes][n_bins]
+    int32_t n_split;           // workgroups per gene; split s takes pairs [s P / n_split, (s + 1) P / n_split)
+    double* split_val;         // [n_genes][n_split][5]: the split's three sums and two minima
+    uint32_t* split_hist;      // [n_genes][n_split][n_bins]
 };
 
-// One workgroup per gene. Thread t takes pairs t, t + 256, ...: the pair's
-// piece sums in piece order give its mean x and mean log x (log2 sum times
-// ln 2, over the gene's n sites); the thread keeps FP64 sums and minima,
-// and the bin of mean log x (the number of edges <= it, less one, clamped
-// to the bins: numpy.digitize and clip) is counted in LDS. A fixed tree
-// over the 256 threads joins the sums and minima, which thread 0 adds to
-// the running totals. Genes of fewer than 2 sites are left unchanged.
+// Workgroup (g, s) reduces split s of the pairs for gene g. Thread t takes
+// the split's pairs t, t + 256, ...: the pair's piece sums in piece order
+// give its mean x and mean log x (log2 sum times ln 2, over the gene's n
+// sites); the thread keeps FP64 sums and minima, and the bin of mean log x
+// (the number of edges <= it, less one, clamped to the bins: numpy.digitize
+// and clip) is counted in LDS. A fixed tree over the 256 threads joins the
+// sums and minima into the split's partials; gene_combine_kernel adds the
+// splits in order to the running totals. Genes of fewer than 2 sites are
+// left unchanged.
 __global__ __launch_bounds__(256) void gene_reduce_kernel(GeneReduceArgs a)
 {
     constexpr double kLn2 = 0.69314718055994530942;
-    const int g = blockIdx.x;
+    const int g = blockIdx.x, sp = blockIdx.y;
     const int t = threadIdx.x;
     const int64_t n = a.gene_n[g];
     if (n < 2) return;
@@ -60,10 +67,11 @@ __global__ __launch_bounds__(256) void gene_reduce_kernel(GeneReduceArgs a)
     for (int i = t; i < a.n_bins; i += 256) s_hist[i] = 0u;
     __syncthreads();
     const int32_t pb = a.gene_pb[g], pe = a.gene_pe[g];
+    const int64_t p0 = a.P * sp / a.n_split, p1 = a.P * (sp + 1) / a.n_split;
     const double inv_n = 1.0 / (double)n;
     double sl = 0.0, sg = 0.0, sq = 0.0;
     double ml = __builtin_inf(), mg = __builtin_inf();
-    for (int64_t p = t; p < a.P; p += 256) {
+    for (int64_t p = p0 + t; p < p1; p += 256) {
         double xl = 0.0, xg = 0.0;
         for (int32_t k = pb; k < pe; ++k) {
             const double2 d = a.part[(size_t)k * (size_t)a.P + (size_t)p];
@@ -104,15 +112,41 @@ __global__ __launch_bounds__(256) void gene_reduce_kernel(GeneReduceArgs a)
         }
         __syncthreads();
     }
+    const size_t at = (size_t)g * (size_t)a.n_split + (size_t)sp;
+    if (t < 5) a.split_val[at * 5 + t] = s_red[t][0];
+    for (int i = t; i < a.n_bins; i += 256) a.split_hist[at * (size_t)a.n_bins + i] = s_hist[i];
+}
+
+// Per gene: the splits' partials in split order added to the running
+// totals (count += P for the launch's pairs).
+__global__ __launch_bounds__(256) void gene_combine_kernel(GeneReduceArgs a)
+{
+    const int g = blockIdx.x;
+    const int t = threadIdx.x;
+    if (a.gene_n[g] < 2) return;
+    const size_t at = (size_t)g * (size_t)a.n_split;
     if (t == 0) {
+        double sl = a.lin_sum[g], sg = a.log_sum[g], sq = a.log_sq_sum[g], ml = a.min_lin[g], mg = a.min_log[g];
+        for (int s = 0; s < a.n_split; ++s) {
+            const double* v = a.split_val + (at + s) * 5;
+            sl += v[0];
+            sg += v[1];
+            sq += v[2];
+            ml = fmin(ml, v[3]);
+            mg = fmin(mg, v[4]);
+        }
         a.count[g] += a.P;
-        a.lin_sum[g] += s_red[0][0];
-        a.log_sum[g] += s_red[1][0];
-        a.log_sq_sum[g] += s_red[2][0];
-        a.min_lin[g] = fmin(a.min_lin[g], s_red[3][0]);
-        a.min_log[g] = fmin(a.min_log[g], s_red[4][0]);
+        a.lin_sum[g] = sl;
+        a.log_sum[g] = sg;
+        a.log_sq_sum[g] = sq;
+        a.min_lin[g] = ml;
+        a.min_log[g] = mg;
     }
-    for (int i = t; i < a.n_bins; i += 256) a.hist[(size_t)g * (size_t)a.n_bins + i] += (int64_t)s_hist[i];
+    for (int i = t; i < a.n_bins; i += 256) {
+        int64_t h = 0;
+        for (int s = 0; s < a.n_split; ++s) h += a.split_hist[(at + s) * (size_t)a.n_bins + i];
+        a.hist[(size_t)g * (size_t)a.n_bins + i] += h;
+    }
 }
 
 }  // namespace gsmc

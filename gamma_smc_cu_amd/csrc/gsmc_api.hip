@@ -348,7 +348,7 @@ struct gsmc_context {
     hipEvent_t ev_stage[2] = {nullptr, nullptr};   // staging buffer filled (copy -> host)
     std::unique_ptr<gsmc_batch> dev_batch;
     DevBuf sum_pairs, sum_site;  // per-site summary path: pair chunk, outputs and running values
-    DevBuf gene_pairs, gene_part, gene_meta, gene_acc;  // per-gene statistics path
+    DevBuf gene_pairs, gene_part, gene_meta, gene_acc, gene_split;  // per-gene statistics path
     void* scratch = nullptr;     // this device's `scratch` (gsmc_run_fb_device_noalloc)
 
     ~gsmc_context();
@@ -1041,9 +1041,12 @@ void check_codes(int flags, const std::vector<Block>& blocks, int64_t row0)
 
 void check_pairs(const gsmc_context* ctx, const int32_t* pi, const int32_t* pj, int64_t P)
 {
+    // the message is built only for a bad pair (building it for every pair
+    // took 18 ms of a 499,500-pair call)
     for (int64_t p = 0; p < P; ++p)
-        require(pi[p] >= 0 && pi[p] < ctx->n && pj[p] >= 0 && pj[p] < ctx->n,
-                "pair index out of range: (" + std::to_string(pi[p]) + ", " + std::to_string(pj[p]) + ")");
+        if (!(pi[p] >= 0 && pi[p] < ctx->n && pj[p] >= 0 && pj[p] < ctx->n))
+            throw Error(GSMC_EINVAL,
+                        "pair index out of range: (" + std::to_string(pi[p]) + ", " + std::to_string(pj[p]) + ")");
 }
 
 // Parallel host copy of staged rows into the caller's [S][P] arrays.
@@ -1740,9 +1743,9 @@ static void plan_genes(int64_t S, int64_t core, int64_t n_genes, const int64_t* 
     std::vector<std::pair<int64_t, int>> ev;  // (site, +1 gene starts / -1 gene ends)
     ev.reserve((size_t)n_genes * 2);
     for (int64_t g = 0; g < n_genes; ++g) {
-        require(lo[g] >= 0 && lo[g] <= hi[g] && hi[g] <= S,
-                "gene " + std::to_string(g) + " site range [" + std::to_string(lo[g]) + ", " +
-                    std::to_string(hi[g]) + ") is not within [0, " + std::to_string(S) + "]");
+        if (!(lo[g] >= 0 && lo[g] <= hi[g] && hi[g] <= S))
+            throw Error(GSMC_EINVAL, "gene " + std::to_string(g) + " site range [" + std::to_string(lo[g]) + ", " +
+                                         std::to_string(hi[g]) + ") is not within [0, " + std::to_string(S) + "]");
         if (hi[g] > lo[g]) {
             ev.emplace_back(lo[g], 1);
             ev.emplace_back(hi[g], -1);
@@ -1852,6 +1855,13 @@ int gsmc_run_fb_gene_stats(gsmc_context* ctx, const int32_t* pi, const int32_t* 
         ctx->gene_acc.ensure(acc_bytes, ctx->device, true);
         ctx->gene_pairs.ensure((size_t)chunk * 2 * sizeof(int32_t), ctx->device, true);
         ctx->gene_part.ensure((size_t)chunk * (size_t)n_pieces * sizeof(double2), ctx->device, true);
+        // workgroups per gene in the reduction: about 2048 in all, each over
+        // at least 2048 pairs of the largest launch
+        const int32_t n_split = (int32_t)std::max<int64_t>(
+            1, std::min<int64_t>((2048 + n_genes - 1) / n_genes, std::max<int64_t>(1, chunk / 2048)));
+        require(n_split <= 65535, "internal: gene reduction split");
+        ctx->gene_split.ensure((size_t)n_genes * (size_t)n_split * (5 * sizeof(double) + (size_t)n_bins * 4),
+                               ctx->device, true);
         char* meta = ctx->gene_meta.as<char>();
         char* acc = ctx->gene_acc.as<char>();
         auto h2d = [&](void* dst, const void* src, size_t bytes) {
@@ -1884,6 +1894,9 @@ int gsmc_run_fb_gene_stats(gsmc_context* ctx, const int32_t* pi, const int32_t* 
         gr.min_lin = reinterpret_cast<double*>(acc + a_mlin);
         gr.min_log = reinterpret_cast<double*>(acc + a_mlog);
         gr.hist = reinterpret_cast<int64_t*>(acc + a_hist);
+        gr.n_split = n_split;
+        gr.split_val = ctx->gene_split.as<double>();
+        gr.split_hist = reinterpret_cast<uint32_t*>(gr.split_val + (size_t)n_genes * (size_t)n_split * 5);
 
         int32_t* d_pi = ctx->gene_pairs.as<int32_t>();
         int32_t* d_pj = d_pi + chunk;
@@ -1902,7 +1915,9 @@ int gsmc_run_fb_gene_stats(gsmc_context* ctx, const int32_t* pi, const int32_t* 
             js.gene_part = ctx->gene_part.as<double2>();
             launch_plan(ctx->dev_batch.get(), {js}, kGeneLaunch, ctx->compute);
             gr.P = n;
-            hipLaunchKernelGGL(gene_reduce_kernel, dim3((unsigned)n_genes), dim3(256), 0, ctx->compute, gr);
+            hipLaunchKernelGGL(gene_reduce_kernel, dim3((unsigned)n_genes, (unsigned)n_split), dim3(256), 0,
+                               ctx->compute, gr);
+            hipLaunchKernelGGL(gene_combine_kernel, dim3((unsigned)n_genes), dim3(256), 0, ctx->compute, gr);
             HIP_OK(hipGetLastError());
         }
         auto d2h = [&](void* dst, const void* src, size_t bytes) {
