@@ -149,4 +149,42 @@ def summarize(acc):
             "min_tmrca": np.where(np.isfinite(acc["min_lin"]), acc["min_lin"], np.nan)}
 
 
-__all__ = ["DEFAULT_BIN_EDGES", "new_accumulators", "gene_site_ranges", "gene_stats", "summarize"]
+def write_results(acc, genes, csv_path=None, npz_path=None, n_haplotypes=None, n_pairs_total=None):
+    """Write the reference pipeline's per-gene outputs
+    (``infer_chromosome.py:239-281``): a CSV with ``gene_id, gene_name,
+    start, end, geom_mean_tmrca, arith_mean_tmrca, min_tmrca, n_pairs,
+    n_sites`` (means to two decimals, blank where a gene has no pairs) and a
+    compressed NPZ of the raw accumulators for offline re-aggregation.
+    ``genes`` are ``(gene_id, gene_name, start, end)`` rows; ``n_pairs_total``
+    defaults to the largest per-gene pair count."""
+    import csv
+
+    rows = [tuple(g) for g in genes]
+    if any(len(g) != 4 for g in rows):
+        raise ValueError("genes must be (gene_id, gene_name, start, end) rows")
+    summ = summarize(acc)
+    n_sites = acc.get("n_sites_per_gene", np.zeros(len(rows), np.int32))
+    fmt = lambda v: f"{v:.2f}" if np.isfinite(v) else ""
+    if csv_path is not None:
+        with open(csv_path, "w", newline="") as fh:
+            w = csv.writer(fh)
+            w.writerow(["gene_id", "gene_name", "start", "end", "geom_mean_tmrca", "arith_mean_tmrca", "min_tmrca",
+                        "n_pairs", "n_sites"])
+            for gi, (gid, name, start, end) in enumerate(rows):
+                w.writerow([gid, name, start, end, fmt(summ["geom_mean_tmrca"][gi]),
+                            fmt(summ["arith_mean_tmrca"][gi]), fmt(acc["min_lin"][gi]), int(acc["count"][gi]),
+                            int(n_sites[gi])])
+    if npz_path is not None:
+        np.savez_compressed(
+            npz_path,
+            gene_id=np.array([g[0] for g in rows]), gene_name=np.array([g[1] for g in rows]),
+            start=np.array([g[2] for g in rows], dtype=np.int64), end=np.array([g[3] for g in rows], dtype=np.int64),
+            count=acc["count"], lin_sum=acc["lin_sum"], log_sum=acc["log_sum"], log_sq_sum=acc["log_sq_sum"],
+            min_lin=acc["min_lin"], min_log=acc["min_log"], histogram=acc["histogram"], bin_edges=acc["bin_edges"],
+            n_sites_per_gene=np.asarray(n_sites, np.int32),
+            n_haplotypes=np.int64(-1 if n_haplotypes is None else n_haplotypes),
+            n_pairs_total=np.int64(n_pairs_total if n_pairs_total is not None
+                                   else (int(acc["count"].max()) if len(rows) else 0)))
+
+
+__all__ = ["DEFAULT_BIN_EDGES", "new_accumulators", "gene_site_ranges", "gene_stats", "summarize", "write_results"]

@@ -154,3 +154,30 @@ def test_run_fb_gene_stats_checks_accumulators_before_the_library():
             ctx.run_fb_gene_stats([(0, 1)], ranges, accumulators=acc)
     with pytest.raises(ValueError, match="histogram"):
         ctx.run_fb_gene_stats([(0, 1)], ranges, accumulators=GS.new_accumulators(2, np.linspace(0, 1, 4)))
+
+
+def test_write_results_matches_the_pipeline_layout(tmp_path):
+    """CSV columns, two-decimal means, blanks for genes without pairs, and
+    the NPZ fields of infer_chromosome.py:239-281 (read back without
+    pickle)."""
+    import csv
+
+    genes = [("ENSG1", "AAA", 100, 900), ("ENSG2", "BBB", 1000, 1001)]
+    acc = GS.new_accumulators(2)
+    acc["count"][0] = 4
+    acc["lin_sum"][0] = 4 * 1234.5
+    acc["log_sum"][0] = 4 * math.log(1000.0)
+    acc["min_lin"][0] = 800.25
+    acc["n_sites_per_gene"] = np.array([12, 1], np.int32)
+    GS.write_results(acc, genes, tmp_path / "pop.csv", tmp_path / "pop.npz", n_haplotypes=8, n_pairs_total=4)
+    rows = list(csv.reader(open(tmp_path / "pop.csv")))
+    assert rows[0] == ["gene_id", "gene_name", "start", "end", "geom_mean_tmrca", "arith_mean_tmrca", "min_tmrca",
+                       "n_pairs", "n_sites"]
+    assert rows[1] == ["ENSG1", "AAA", "100", "900", "1000.00", "1234.50", "800.25", "4", "12"]
+    assert rows[2] == ["ENSG2", "BBB", "1000", "1001", "", "", "", "0", "1"]
+    z = np.load(tmp_path / "pop.npz")
+    assert z["gene_id"].tolist() == ["ENSG1", "ENSG2"] and z["start"].tolist() == [100, 1000]
+    assert np.array_equal(z["histogram"], acc["histogram"]) and np.array_equal(z["bin_edges"], O.GENE_BIN_EDGES)
+    assert int(z["n_haplotypes"]) == 8 and int(z["n_pairs_total"]) == 4 and z["count"].tolist() == [4, 0]
+    with pytest.raises(ValueError, match="gene_id"):
+        GS.write_results(acc, [(100, 900)], tmp_path / "x.csv")
